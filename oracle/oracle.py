@@ -1,0 +1,211 @@
+"""ctypes view of the CPU restatement in gs_oracle.c, plus helpers to run the
+reference binary built by oracle/Makefile.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+import tempfile
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_gs.so")
+REF_BIN = os.path.join(HERE, "_ref", "gramsci")
+REF_BIN_FAST = os.path.join(HERE, "_ref", "gramsci_fast")
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("rmin", ctypes.c_double), ("rmax", ctypes.c_double),
+                ("nbins", ctypes.c_int32), ("nmu", ctypes.c_int32), ("log_bins", ctypes.c_int32)]
+
+
+@dataclass
+class Params:
+    rmin: float
+    rmax: float
+    nbins: int
+    nmu: int = 1
+    log_bins: bool = False
+
+    def c(self) -> _Params:
+        return _Params(self.rmin, self.rmax, self.nbins, self.nmu, int(bool(self.log_bins)))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.check_call(["make", "-s", "-C", HERE, "liboracle_gs.so"])
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        dp, i32p, u64p = P(ctypes.c_double), P(ctypes.c_int32), P(ctypes.c_uint64)
+        pp = P(_Params)
+        L.or_radial_bins.argtypes = [pp, dp]
+        L.or_normalise.argtypes = [dp, ctypes.c_int64, ctypes.c_int64, dp]
+        L.or_pair_rbin.argtypes = [pp, dp, dp]
+        L.or_pair_mubin.argtypes = [pp, dp, dp]
+        L.or_2pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, u64p]
+        L.or_3pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, u64p]
+        L.or_4pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, dp, u64p]
+        for f in ("or_2pcf", "or_3pcf", "or_4pcf", "or_pair_rbin", "or_pair_mubin",
+                  "or_n3cfg", "or_n4cfg", "or_bintable3", "or_bintable4"):
+            getattr(L, f).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _prep(xyz, w, buf):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    assert xyz.ndim == 2 and xyz.shape[1] == 3
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    buf = None if buf is None else np.ascontiguousarray(buf, dtype=np.int32)
+    return xyz, w, buf
+
+
+def radial_bins(p: Params) -> np.ndarray:
+    e = np.zeros(p.nbins + 1)
+    lib().or_radial_bins(ctypes.byref(p.c()), _ptr(e, ctypes.c_double))
+    return e
+
+
+def normalise(w, n_data, n_rand):
+    wn = np.zeros(n_data + n_rand)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+    lib().or_normalise(_ptr(w, ctypes.c_double), n_data, n_rand, _ptr(wn, ctypes.c_double))
+    return wn
+
+
+def pair_rbin(p: Params, xi, xj) -> int:
+    a = np.ascontiguousarray(xi, dtype=np.float64)
+    b = np.ascontiguousarray(xj, dtype=np.float64)
+    return lib().or_pair_rbin(ctypes.byref(p.c()), _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double))
+
+
+def pair_mubin(p: Params, xi, xj) -> int:
+    a = np.ascontiguousarray(xi, dtype=np.float64)
+    b = np.ascontiguousarray(xj, dtype=np.float64)
+    return lib().or_pair_mubin(ctypes.byref(p.c()), _ptr(a, ctypes.c_double), _ptr(b, ctypes.c_double))
+
+
+def n3cfg(nbins):
+    return lib().or_n3cfg(nbins)
+
+
+def n4cfg(nbins):
+    return lib().or_n4cfg(nbins)
+
+
+def pcf2(xyz, w, n_data, p: Params, buf=None):
+    """Returns dict(cnt[3,nb,nm] u64, wsum[3,nb,nm], n2[nb,nm], n3[nb,nm], overflow)."""
+    xyz, w, buf = _prep(xyz, w, buf)
+    n = xyz.shape[0]
+    cnt = np.zeros((3, p.nbins, p.nmu), np.uint64)
+    wsum = np.zeros((3, p.nbins, p.nmu))
+    n2 = np.zeros((p.nbins, p.nmu))
+    n3 = np.zeros((p.nbins, p.nmu))
+    ov = ctypes.c_uint64(0)
+    rc = lib().or_2pcf(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
+                       n_data, n - n_data, ctypes.byref(p.c()), _ptr(cnt, ctypes.c_uint64),
+                       _ptr(wsum, ctypes.c_double), _ptr(n2, ctypes.c_double), _ptr(n3, ctypes.c_double),
+                       ctypes.byref(ov))
+    if rc:
+        raise ValueError(f"or_2pcf failed: {rc}")
+    return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, overflow=int(ov.value))
+
+
+def pcf3(xyz, w, n_data, p: Params, buf=None):
+    """Returns dict(cnt[ncfg,4] u64, n2[ncfg], n3[ncfg], n_cand)."""
+    xyz, w, buf = _prep(xyz, w, buf)
+    n = xyz.shape[0]
+    nc = n3cfg(p.nbins)
+    cnt = np.zeros((nc, 4), np.uint64)
+    n2 = np.zeros(nc)
+    n3 = np.zeros(nc)
+    cand = ctypes.c_uint64(0)
+    rc = lib().or_3pcf(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
+                       n_data, n - n_data, ctypes.byref(p.c()), _ptr(cnt, ctypes.c_uint64),
+                       _ptr(n2, ctypes.c_double), _ptr(n3, ctypes.c_double), ctypes.byref(cand))
+    if rc:
+        raise ValueError(f"or_3pcf failed: {rc}")
+    return dict(cnt=cnt, n2=n2, n3=n3, n_cand=int(cand.value))
+
+
+def pcf4(xyz, w, n_data, p: Params, buf=None):
+    """Returns dict(cnt[ncfg,nb,nb,16], n2/n3[ncfg,nb,nb], disc_n/disc_r[6,ncfg,nb,nb], n_quads)."""
+    xyz, w, buf = _prep(xyz, w, buf)
+    n = xyz.shape[0]
+    nc, nb = n4cfg(p.nbins), p.nbins
+    cnt = np.zeros((nc, nb, nb, 16), np.uint64)
+    n2 = np.zeros((nc, nb, nb))
+    n3 = np.zeros((nc, nb, nb))
+    dn = np.zeros((6, nc, nb, nb))
+    dr = np.zeros((6, nc, nb, nb))
+    q = ctypes.c_uint64(0)
+    rc = lib().or_4pcf(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
+                       n_data, n - n_data, ctypes.byref(p.c()), _ptr(cnt, ctypes.c_uint64),
+                       _ptr(n2, ctypes.c_double), _ptr(n3, ctypes.c_double),
+                       _ptr(dn, ctypes.c_double), _ptr(dr, ctypes.c_double), ctypes.byref(q))
+    if rc:
+        raise ValueError(f"or_4pcf failed: {rc}")
+    return dict(cnt=cnt, n2=n2, n3=n3, disc_n=dn, disc_r=dr, n_quads=int(q.value))
+
+
+# ------------------------------------------------------------------ reference binary
+
+def write_catalogue(path, xyz, w=None, extra=None):
+    """ASCII x y z w [extra] with 17 significant digits (exact round trip)."""
+    xyz = np.asarray(xyz, dtype=np.float64)
+    w = np.ones(len(xyz)) if w is None else np.asarray(w, dtype=np.float64)
+    cols = [xyz[:, 0], xyz[:, 1], xyz[:, 2], w]
+    fmt = ["%.17g"] * 4
+    if extra is not None:
+        cols.append(np.asarray(extra))
+        fmt.append("%d")
+    np.savetxt(path, np.column_stack(cols), fmt=fmt)
+
+
+def run_reference(args, cwd, binary=REF_BIN, threads=None, timeout=3600):
+    """Runs the reference program with the given argument list; returns (stdout, wall_s)."""
+    import time
+    env = dict(os.environ)
+    if threads is not None:
+        env["OMP_NUM_THREADS"] = str(threads)
+    t0 = time.perf_counter()
+    r = subprocess.run([binary] + list(args), cwd=cwd, env=env, capture_output=True, text=True,
+                       timeout=timeout)
+    dt = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError(f"reference failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    return r.stdout, dt
+
+
+def reference_table(xyz_d, w_d, xyz_r, w_r, p: Params, mode="-2pcf", threads=None, binary=REF_BIN):
+    """Runs the reference on the catalogues and returns the numeric rows of its output file."""
+    with tempfile.TemporaryDirectory() as td:
+        write_catalogue(os.path.join(td, "d.txt"), xyz_d, w_d)
+        args = ["-gal", "d.txt"]
+        if xyz_r is not None and len(xyz_r):
+            write_catalogue(os.path.join(td, "r.txt"), xyz_r, w_r)
+            args += ["-ran", "r.txt"]
+        args += ["-rmin", repr(float(p.rmin)), "-rmax", repr(float(p.rmax)), "-nbins", str(p.nbins),
+                 "-nmu", str(p.nmu), "-out", "o.txt"]
+        if p.log_bins:
+            args.append("-log")
+        args.append(mode)
+        run_reference(args, td, binary=binary, threads=threads)
+        with open(os.path.join(td, "o.txt")) as f:
+            text = f.read()
+    return text

@@ -1,0 +1,740 @@
+// api.cpp -- the C ABI (include/gramsci_hip.h): contexts, catalogue upload,
+// grid build, exact bin thresholds, kernel launches and the host-side
+// combination of exact class sums into the reference's N2 / N3.
+#include "../../include/gramsci_hip.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gs_kernels.h"
+#include "gs_api_internal.h"
+
+using namespace gs;
+
+namespace gs {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(GS_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------ device buffers
+template <class T>
+int DevBuf<T>::reserve(size_t count)
+{
+    if (count <= cap) return GS_OK;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    size_t c = std::max<size_t>(count, 1);
+    hipError_t e = hipMalloc((void**)&ptr, c * sizeof(T));
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        return fail(GS_ENOMEM, "hipMalloc(%zu bytes): %s", c * sizeof(T), hipGetErrorString(e));
+    }
+    cap = c;
+    return GS_OK;
+}
+
+template <class T>
+void DevBuf<T>::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+
+template struct DevBuf<double>;
+template struct DevBuf<int32_t>;
+template struct DevBuf<uint32_t>;
+template struct DevBuf<double4>;
+template struct DevBuf<unsigned long long>;
+template struct DevBuf<unsigned char>;
+
+// ------------------------------------------------------------ thresholds
+// The reference's radial bin of a squared distance s (gramsci.F90:136-145,
+// :334-338), 1-based, evaluated with the host libm exactly as written.
+static int ref_bin(const gs_params& p, double s)
+{
+    const double d = std::sqrt(s);
+    if (p.log_bins) {
+        const double delta_r = (std::log10(p.rmax) - std::log10(p.rmin)) / (double)p.nbins;
+        const double inv_delta_r = 1.0 / delta_r;
+        const double log_rmin = std::log10(p.rmin);
+        return (int)std::floor((std::log10(d) - log_rmin) * inv_delta_r) + 1;
+    }
+    const double delta_r = (p.rmax - p.rmin) / (double)p.nbins;
+    const double inv_delta_r = 1.0 / delta_r;
+    return (int)std::floor((d - p.rmin) * inv_delta_r) + 1;
+}
+
+static inline uint64_t dbits(double x)
+{
+    uint64_t u;
+    memcpy(&u, &x, 8);
+    return u;
+}
+static inline double bitsd(uint64_t u)
+{
+    double x;
+    memcpy(&x, &u, 8);
+    return x;
+}
+
+// smallest non-negative double s in [lo, hi] (bit patterns) with pred(s) true; pred monotone false->true.
+template <class F>
+static uint64_t first_true(uint64_t lo, uint64_t hi, F pred)
+{
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (pred(bitsd(mid))) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+int check_params(const gs_params* p, int mode)
+{
+    if (!p) return fail(GS_EINVAL, "params is NULL");
+    if (p->nbins < 1 || p->nbins > kMaxBins) return fail(GS_EINVAL, "nbins must be in 1..127 (got %d)", p->nbins);
+    if (p->nmu < 1 || p->nmu > kMaxBins) return fail(GS_EINVAL, "nmu must be in 1..127 (got %d)", p->nmu);
+    if (!(p->rmin >= 0.0) || !(p->rmax > p->rmin) || !std::isfinite(p->rmax))
+        return fail(GS_EINVAL, "need 0 <= rmin < rmax (got rmin=%g rmax=%g)", p->rmin, p->rmax);
+    if (p->log_bins && !(p->rmin > 0.0)) return fail(GS_EINVAL, "log bins need rmin > 0");
+    if (p->precision != GS_FP64 && p->precision != GS_FP32_GUARD)
+        return fail(GS_EINVAL, "precision must be GS_FP64 or GS_FP32_GUARD");
+    if (p->cell_div < 0 || p->cell_div > 16) return fail(GS_EINVAL, "cell_div must be in 0..16");
+    if (mode >= 3 && p->nmu != 1) return fail(GS_EINVAL, "3PCF/4PCF support nmu = 1 only");
+    return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" int gs_bin_thresholds(const gs_params* p, double* t)
+{
+    int rc = check_params(p, 2);
+    if (rc) return rc;
+    if (!t) return fail(GS_EINVAL, "t is NULL");
+    const gs_params P = *p;
+    const int nb = P.nbins;
+    const double rmax2 = P.rmax * P.rmax;
+    const uint64_t top = dbits(rmax2 * 4.0 + 1.0);
+    // t[0]: largest s with sqrt(s) <= rmin
+    uint64_t u = first_true(0, top, [&](double s) { return std::sqrt(s) > P.rmin; });
+    t[0] = bitsd(u - (u > 0 ? 1 : 0));
+    if (std::sqrt(t[0]) > P.rmin) t[0] = 0.0;
+    // t[nb+1]: smallest s not kept by the r-ball (s <= rmax^2) and the strict d < rmax
+    const uint64_t hi = first_true(dbits(t[0]), top, [&](double s) { return !(s <= rmax2 && std::sqrt(s) < P.rmax); });
+    t[nb + 1] = bitsd(hi);
+    const uint64_t lo = dbits(t[0]) + 1;
+    for (int k = 1; k <= nb; ++k) {
+        uint64_t v = first_true(lo, hi, [&](double s) { return ref_bin(P, s) >= k + 1; });
+        t[k] = bitsd(v);
+    }
+    // the bin must be a non-decreasing step function of s near each edge, with
+    // bin >= 1 just above rmin; verify in a window of ulps around every threshold
+    const int win = 256;
+    if (lo < hi && ref_bin(P, bitsd(lo)) < 1) return fail(GS_EINVAL, "bin formula gives 0 just above rmin");
+    for (int k = 0; k <= nb + 1; ++k) {
+        const uint64_t c = dbits(t[k]);
+        int prev = -1000000;
+        for (int64_t d = -win; d <= win; ++d) {
+            const int64_t b = (int64_t)c + d;
+            if (b <= (int64_t)lo || b >= (int64_t)hi) continue;
+            const int v = ref_bin(P, bitsd((uint64_t)b));
+            if (v < prev) return fail(GS_EINVAL, "bin formula not monotone near threshold %d", k);
+            prev = v;
+        }
+    }
+    return GS_OK;
+}
+
+namespace gs {
+
+// ------------------------------------------------------------ FP32 guard band
+// Upper bound of |s32 - s| for a pair whose cell-relative FP32 coordinates are
+// bounded by rrel per component (the squared distance s near t).
+static double s_guard(double t, double ew)
+{
+    const double root = std::sqrt(std::max(t, 0.0));
+    return 2.0 * (2.0 * std::sqrt(3.0) * root * ew + 3.0 * ew * ew + std::ldexp(t, -22));
+}
+
+static float f_up(double x)
+{
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+static float f_down(double x)
+{
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
+int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B)
+{
+    memset(&B, 0, sizeof(B));
+    int rc = gs_bin_thresholds(&p, B.t64);
+    if (rc) return rc;
+    const int nb = p.nbins;
+    B.nb = nb;
+    B.nm = p.nmu;
+    B.log_bins = p.log_bins;
+    B.rsd = p.nmu >= 2;
+    B.mu_scale = (double)(0.5f * (float)p.nmu);
+    B.ms32 = (float)B.mu_scale;
+    // error of one cell-relative coordinate difference (FP64 -> FP32 rounding of both ends + the FP32 subtract)
+    const double ew = std::ldexp(2.0 * rrel + p.rmax, -24) * 1.01;
+    for (int b = 0; b < nb; ++b) {
+        const double lo = b == 0 ? B.t64[0] : B.t64[b];
+        const double hi = b == nb - 1 ? std::min(B.t64[nb], B.t64[nb + 1]) : B.t64[b + 1];
+        B.ok_lo[b] = f_up(lo + s_guard(lo, ew));
+        B.ok_hi[b] = f_down(hi - s_guard(hi, ew));
+    }
+    B.rej_lo = f_down(B.t64[0] - s_guard(B.t64[0], ew));
+    B.rej_hi = f_up(B.t64[nb + 1] + s_guard(B.t64[nb + 1], ew));
+    if (p.log_bins) {
+        const double delta = (std::log10(p.rmax) - std::log10(p.rmin)) / nb;
+        const double inv = 1.0 / delta;
+        B.ua = (float)(0.5 * std::log10(2.0) * inv);
+        B.ub = (float)(-std::log10(p.rmin) * inv);
+    } else {
+        const double inv = 1.0 / ((p.rmax - p.rmin) / nb);
+        B.ua = (float)inv;
+        B.ub = (float)(-p.rmin * inv);
+    }
+    // mu guard: per-component errors of sep (ew) and of the absolute midpoint (em)
+    const double em = std::ldexp(2.0 * xmax + 3.0 * rrel, -24) * 1.01;
+    B.mu_ew = (float)(2.0 * std::sqrt(3.0) * ew * 4.0);
+    B.mu_em = (float)(2.0 * std::sqrt(3.0) * em * 4.0);
+    B.mu_c = (float)std::ldexp(1.0, -18);
+    // fixed point: the largest pair / triple product stays below 2^61
+    const double wm = std::max(wmax, 1e-300);
+    B.fx_shift2 = 61 - (int)std::ceil(std::log2(wm * wm * (1.0 + 1e-12)));
+    B.fx_shift3 = 61 - (int)std::ceil(std::log2(wm * wm * wm * (1.0 + 1e-12)));
+    if (B.fx_shift2 > 1000) B.fx_shift2 = 1000;
+    if (B.fx_shift3 > 1000) B.fx_shift3 = 1000;
+    return GS_OK;
+}
+
+// Half stencil of x-rows for cells of edge h and a search radius rmax, m = ceil(rmax/h).
+std::vector<Row> half_stencil(double h, double rmax, int m)
+{
+    std::vector<Row> rows;
+    auto gap = [](int d) { return (double)std::max(std::abs(d) - 1, 0); };
+    const double lim = (rmax / h) * (1.0 + 1e-6) + 1e-9;
+    for (int dz = 0; dz <= m; ++dz)
+        for (int dy = -m; dy <= m; ++dy) {
+            if (dz == 0 && dy < 0) continue;
+            const double gyz = gap(dy) * gap(dy) + gap(dz) * gap(dz);
+            if (gyz > lim * lim) continue;
+            int mx = 0;
+            for (int dx = 1; dx <= m; ++dx)
+                if (gyz + gap(dx) * gap(dx) <= lim * lim) mx = dx;
+            Row r;
+            r.dy = (int8_t)dy;
+            r.dz = (int8_t)dz;
+            r.x0 = (int8_t)((dz == 0 && dy == 0) ? 0 : -mx);
+            r.x1 = (int8_t)mx;
+            rows.push_back(r);
+        }
+    return rows;
+}
+
+// 128-bit fixed point (lo, hi) scaled by 2^-shift -> long double
+long double fx_value(unsigned long long lo, unsigned long long hi, int shift)
+{
+    long double v = (long double)hi * 18446744073709551616.0L + (long double)lo;
+    return std::ldexp(v, -shift);
+}
+
+}  // namespace gs
+
+// ====================================================================== context
+
+struct gs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n_cu = 256;
+    // catalogue (original order)
+    int64_t n_data = 0, n_rand = 0;
+    bool has_w = false, has_buf = false;
+    long double wsum_d = 0, wsum_r = 0;
+    double wmax = 1.0;
+    double bbox[6] = {0, 0, 0, 0, 0, 0};
+    DevBuf<double> xyz, w;
+    DevBuf<int32_t> buf;
+    // grid
+    GridDesc g{};
+    bool grid_valid = false;
+    double grid_h = 0.0;
+    DevBuf<int32_t> cell_of, cell_start, fill, chunk_nc, chunk_off, scan_tmp;
+    DevBuf<double4> pw;
+    DevBuf<uint32_t> flags;
+    DevBuf<int32_t> orig;
+    int32_t n_chunks = 0;
+    // launch scratch
+    DevBuf<unsigned char> params;
+    DevBuf<unsigned long long> out;
+    DevBuf<int32_t> queue;
+    DevBuf<double> part;
+    GsExtra extra;
+};
+
+using namespace gs;
+
+static int set_device(gs_ctx* c)
+{
+    hipError_t e = hipSetDevice(c->device);
+    return e == hipSuccess ? GS_OK : hip_fail(e, "hipSetDevice");
+}
+
+extern "C" const char* gs_last_error(void) { return gs::g_err.c_str(); }
+extern "C" const char* gs_version(void) { return "gramsci_hip 0.1 (gfx950)"; }
+
+extern "C" int gs_create(gs_ctx** out, int device)
+{
+    if (!out) return fail(GS_EINVAL, "ctx is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= device || device < 0) return fail(GS_ENODEV, "no HIP device %d", device);
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(GS_ENODEV, "device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
+    gs_ctx* c = new gs_ctx();
+    c->device = device;
+    c->n_cu = prop.multiProcessorCount;
+    if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreate(&c->stream)) != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    for (auto& ev : c->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) {
+            delete c;
+            return hip_fail(e, "hipEventCreate");
+        }
+    *out = c;
+    return GS_OK;
+}
+
+extern "C" int gs_destroy(gs_ctx* c)
+{
+    if (!c) return GS_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->xyz.release(); c->w.release(); c->buf.release();
+    c->cell_of.release(); c->cell_start.release(); c->fill.release(); c->chunk_nc.release();
+    c->chunk_off.release(); c->scan_tmp.release(); c->pw.release(); c->flags.release(); c->orig.release();
+    c->params.release(); c->out.release(); c->queue.release(); c->part.release();
+    extra_release(c->extra);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GS_OK;
+}
+
+static int finish_catalogue(gs_ctx* c)
+{
+    // bounding box on the device
+    const int64_t n = c->n_data + c->n_rand;
+    int rc;
+    const int nblk = 256;
+    if ((rc = c->part.reserve(nblk * 6))) return rc;
+    hipError_t e = launch_bbox(c->xyz.ptr, n, c->part.ptr, nblk, c->stream);
+    if (e != hipSuccess) return hip_fail(e, "bbox kernel");
+    std::vector<double> part(nblk * 6);
+    e = hipMemcpyAsync(part.data(), c->part.ptr, nblk * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "bbox copy");
+    for (int k = 0; k < 3; ++k) { c->bbox[k] = 1e308; c->bbox[3 + k] = -1e308; }
+    for (int b = 0; b < nblk; ++b)
+        for (int k = 0; k < 3; ++k) {
+            c->bbox[k] = std::min(c->bbox[k], part[b * 6 + k]);
+            c->bbox[3 + k] = std::max(c->bbox[3 + k], part[b * 6 + 3 + k]);
+        }
+    if (n == 0)
+        for (int k = 0; k < 6; ++k) c->bbox[k] = 0.0;
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(c->bbox[k])) return fail(GS_EINVAL, "catalogue has non-finite coordinates");
+    c->grid_valid = false;
+    return GS_OK;
+}
+
+static int host_weight_stats(gs_ctx* c, const double* w)
+{
+    c->wsum_d = 0;
+    c->wsum_r = 0;
+    c->wmax = 1.0;
+    const int64_t n = c->n_data + c->n_rand;
+    if (!w) {
+        c->wsum_d = (long double)c->n_data;
+        c->wsum_r = (long double)c->n_rand;
+        return GS_OK;
+    }
+    double wm = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return fail(GS_EINVAL, "weights must be finite and >= 0 (w[%lld]=%g)", (long long)i, w[i]);
+        wm = std::max(wm, w[i]);
+        if (i < c->n_data) c->wsum_d += w[i]; else c->wsum_r += w[i];
+    }
+    c->wmax = wm > 0 ? wm : 1.0;
+    return GS_OK;
+}
+
+extern "C" int gs_set_catalogue(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf, int64_t n_data,
+                                int64_t n_rand)
+{
+    if (!c || (!xyz && n_data + n_rand > 0)) return fail(GS_EINVAL, "NULL context or coordinates");
+    if (n_data < 0 || n_rand < 0 || n_data + n_rand > (int64_t)INT32_MAX - 1024)
+        return fail(GS_EINVAL, "bad catalogue size (n_data=%lld n_rand=%lld)", (long long)n_data, (long long)n_rand);
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int64_t n = n_data + n_rand;
+    c->n_data = n_data;
+    c->n_rand = n_rand;
+    if ((rc = host_weight_stats(c, w))) return rc;
+    c->has_w = w != nullptr;
+    c->has_buf = buf != nullptr;
+    if ((rc = c->xyz.reserve(3 * n))) return rc;
+    hipError_t e = hipMemcpyAsync(c->xyz.ptr, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && w) {
+        if ((rc = c->w.reserve(n))) return rc;
+        e = hipMemcpyAsync(c->w.ptr, w, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    }
+    if (e == hipSuccess && buf) {
+        if ((rc = c->buf.reserve(n))) return rc;
+        e = hipMemcpyAsync(c->buf.ptr, buf, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "catalogue upload");
+    return finish_catalogue(c);
+}
+
+extern "C" int gs_set_catalogue_device(gs_ctx* c, const double* d_xyz, const double* d_w, const int32_t* d_buf,
+                                       int64_t n_data, int64_t n_rand)
+{
+    if (!c || (!d_xyz && n_data + n_rand > 0)) return fail(GS_EINVAL, "NULL context or coordinates");
+    if (n_data < 0 || n_rand < 0 || n_data + n_rand > (int64_t)INT32_MAX - 1024)
+        return fail(GS_EINVAL, "bad catalogue size");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int64_t n = n_data + n_rand;
+    c->n_data = n_data;
+    c->n_rand = n_rand;
+    c->has_w = d_w != nullptr;
+    c->has_buf = d_buf != nullptr;
+    if ((rc = c->xyz.reserve(3 * n))) return rc;
+    hipError_t e = hipMemcpyAsync(c->xyz.ptr, d_xyz, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+    std::vector<double> hw;
+    if (e == hipSuccess && d_w) {
+        if ((rc = c->w.reserve(n))) return rc;
+        e = hipMemcpyAsync(c->w.ptr, d_w, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
+        hw.resize(n);
+        if (e == hipSuccess) e = hipMemcpyAsync(hw.data(), d_w, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess && d_buf) {
+        if ((rc = c->buf.reserve(n))) return rc;
+        e = hipMemcpyAsync(c->buf.ptr, d_buf, n * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "catalogue copy");
+    if ((rc = host_weight_stats(c, d_w ? hw.data() : nullptr))) return rc;
+    return finish_catalogue(c);
+}
+
+namespace gs {
+
+// Picks the cell edge: rmax / m, widened if the grid would exceed kMaxCells.
+static double choose_h(const gs_ctx* c, double rmax, int m)
+{
+    const int64_t kMaxCells = 1 << 24;
+    double h = rmax / m;
+    double ext = 0;
+    for (int k = 0; k < 3; ++k) ext = std::max(ext, c->bbox[3 + k] - c->bbox[k]);
+    for (;;) {
+        int64_t nc = 1;
+        for (int k = 0; k < 3; ++k) nc *= (int64_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
+        if (nc <= kMaxCells) break;
+        h *= 1.25;
+    }
+    (void)ext;
+    return h;
+}
+
+int build_grid(gs_ctx* c, double h)
+{
+    if (c->grid_valid && c->grid_h == h) return GS_OK;
+    const int64_t n = c->n_data + c->n_rand;
+    GridDesc g{};
+    g.h = h;
+    int64_t ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+        g.lo[k] = c->bbox[k];
+        g.dim[k] = (int32_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
+        ncell *= g.dim[k];
+    }
+    if (ncell > (1 << 26)) return fail(GS_ELIMIT, "grid too large (%lld cells)", (long long)ncell);
+    g.ncell = (int32_t)ncell;
+    int rc;
+    if ((rc = c->cell_of.reserve(n)) || (rc = c->cell_start.reserve(ncell + 1)) || (rc = c->fill.reserve(ncell + 1)) ||
+        (rc = c->chunk_nc.reserve(ncell + 1)) || (rc = c->chunk_off.reserve(ncell + 1)) ||
+        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) ||
+        (rc = c->flags.reserve(n)) || (rc = c->orig.reserve(n)))
+        return rc;
+    hipError_t e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
+    if (e == hipSuccess) e = launch_cell_count(c->xyz.ptr, n, g, c->cell_of.ptr, c->fill.ptr, c->stream);
+    if (e == hipSuccess) e = exclusive_scan(c->fill.ptr, c->cell_start.ptr, ncell + 1, c->scan_tmp.ptr, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
+    if (e == hipSuccess)
+        e = launch_scatter(c->xyz.ptr, c->has_w ? c->w.ptr : nullptr, c->has_buf ? c->buf.ptr : nullptr, n, c->n_data,
+                           c->cell_of.ptr, c->cell_start.ptr, c->fill.ptr, c->pw.ptr, c->flags.ptr, c->orig.ptr,
+                           c->stream);
+    if (e == hipSuccess) e = launch_chunk_counts(c->cell_start.ptr, (int)ncell, c->chunk_nc.ptr, c->stream);
+    if (e == hipSuccess) e = exclusive_scan(c->chunk_nc.ptr, c->chunk_off.ptr, ncell + 1, c->scan_tmp.ptr, c->stream);
+    int32_t nch = 0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&nch, c->chunk_off.ptr + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "grid build");
+    c->g = g;
+    c->grid_h = h;
+    c->n_chunks = nch;
+    c->grid_valid = true;
+    return GS_OK;
+}
+
+int default_cell_div(const gs_ctx* c, double rmax, int mode)
+{
+    // aim at ~200 points per cell for the pair kernel (one 256-lane chunk per cell)
+    const int64_t n = c->n_data + c->n_rand;
+    double vol = 1.0;
+    for (int k = 0; k < 3; ++k) vol *= std::max(c->bbox[3 + k] - c->bbox[k], rmax * 1e-3);
+    const double dens = n / vol;
+    if (mode != 2) return 1;
+    int best = 1;
+    for (int m = 1; m <= 8; ++m) {
+        const double h = rmax / m;
+        if (dens * h * h * h >= 160.0) best = m;
+    }
+    return best;
+}
+
+}  // namespace gs
+
+// ====================================================================== 2PCF
+
+extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2, double* n3,
+                             gs_stats* st)
+{
+    if (!c) return fail(GS_EINVAL, "NULL context");
+    int rc = check_params(p, 2);
+    if (rc) return rc;
+    if ((rc = set_device(c))) return rc;
+    const int nb = p->nbins, nm = p->nmu;
+    const bool rsd = nm >= 2, fp64 = p->precision == GS_FP64, weighted = c->has_w;
+    const int nbt = 3 * nb * nm;
+    const int m = p->cell_div > 0 ? p->cell_div : default_cell_div(c, p->rmax, 2);
+    const double h = choose_h(c, p->rmax, m);
+    const int mst = (int)std::ceil(p->rmax / h - 1e-12);
+    std::vector<Row> rows = half_stencil(h, p->rmax, mst);
+    if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
+
+    (void)hipEventRecord(c->ev[0], c->stream);
+    if ((rc = build_grid(c, h))) return rc;
+    (void)hipEventRecord(c->ev[1], c->stream);
+
+    double xmax = 0;
+    for (int k = 0; k < 6; ++k) xmax = std::max(xmax, std::fabs(c->bbox[k]));
+    BinDesc B;
+    if ((rc = fill_bins(*p, (mst + 1) * h, xmax, c->wmax, B))) return rc;
+
+    // LDS histogram replicas: as many as fit in 32 KB of counters, at most 32
+    int reps = 1;
+    while (reps < 32 && (size_t)nbt * reps * 2 * 4 <= 32 * 1024) reps *= 2;
+    while (reps > 1 && pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted) > 64 * 1024) reps /= 2;
+    if (pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted) > 64 * 1024)
+        return fail(GS_ELIMIT, "histogram of %d bins does not fit in LDS", nbt);
+
+    const size_t nout = (size_t)nbt + 2 * (size_t)nbt + 4;
+    if ((rc = c->params.reserve(sizeof(BinDesc) + rows.size() * sizeof(Row) + 64)) || (rc = c->out.reserve(nout)) ||
+        (rc = c->queue.reserve(8)))
+        return rc;
+    unsigned char* dB = c->params.ptr;
+    Row* dRows = (Row*)(c->params.ptr + ((sizeof(BinDesc) + 15) / 16) * 16);
+    hipError_t e = hipMemcpyAsync(dB, &B, sizeof(B), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && !rows.empty())
+        e = hipMemcpyAsync(dRows, rows.data(), rows.size() * sizeof(Row), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->out.ptr, 0, nout * sizeof(unsigned long long), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->queue.ptr, 0, 8 * sizeof(int32_t), c->stream);
+    if (e != hipSuccess) return hip_fail(e, "parameter upload");
+
+    PairArgs a{};
+    a.pw = c->pw.ptr;
+    a.flags = c->flags.ptr;
+    a.cell_start = c->cell_start.ptr;
+    a.chunk_off = c->chunk_off.ptr;
+    a.rows = dRows;
+    a.bins = (const BinDesc*)dB;
+    a.queue = c->queue.ptr;
+    a.out_cnt = c->out.ptr;
+    a.out_w = c->out.ptr + nbt;
+    a.out_stats = c->out.ptr + 3 * nbt;
+    a.g = c->g;
+    a.g.m = mst;
+    a.nrows = (int)rows.size();
+    a.n_chunks = c->n_chunks;
+    a.reps = reps;
+    const int occ = pair2_occupancy(nb, nm, reps, fp64, rsd, weighted);
+    int nblocks = c->n_cu * occ;
+    if (nblocks > c->n_chunks) nblocks = std::max(c->n_chunks, 1);
+    (void)hipEventRecord(c->ev[2], c->stream);
+    if (c->n_chunks > 0) {
+        e = launch_pair2(a, nb, nm, fp64, rsd, weighted, nblocks, c->stream);
+        if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
+    }
+    (void)hipEventRecord(c->ev[3], c->stream);
+    std::vector<unsigned long long> h_out(nout);
+    e = hipMemcpyAsync(h_out.data(), c->out.ptr, nout * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "pair kernel");
+
+    // outputs: counts [3][nb][nm]; the kernel's layout is (cls*nb + b)*nm + m, identical
+    uint64_t total = 0;
+    for (int k = 0; k < nbt; ++k) {
+        if (cnt) cnt[k] = h_out[k];
+        total += h_out[k];
+    }
+    std::vector<long double> S(nbt);
+    for (int k = 0; k < nbt; ++k)
+        S[k] = weighted ? fx_value(h_out[nbt + 2 * k], h_out[nbt + 2 * k + 1], B.fx_shift2) : (long double)h_out[k];
+    if (wsum)
+        for (int k = 0; k < nbt; ++k) wsum[k] = (double)S[k];
+    const long double Wd = c->wsum_d, Wr = c->wsum_r;
+    for (int b = 0; b < nb * nm; ++b) {
+        const long double dd = S[b], dr = S[nb * nm + b], rr = S[2 * nb * nm + b];
+        long double v2 = 0, v3 = 0;
+        if (Wd > 0) v2 += dd / (Wd * Wd);
+        if (Wd > 0 && Wr > 0) v2 -= dr / (Wd * Wr);
+        if (Wr > 0) { v2 += rr / (Wr * Wr); v3 = rr / (Wr * Wr); }
+        if (n2) n2[b] = (double)v2;
+        if (n3) n3[b] = (double)v3;
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        float ms_grid = 0, ms_count = 0;
+        (void)hipEventElapsedTime(&ms_grid, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&ms_count, c->ev[2], c->ev[3]);
+        st->ms_grid = ms_grid;
+        st->ms_count = ms_count;
+        st->ms_kernel_avg = ms_count;
+        st->n_units = total;
+        st->n_binned = total;
+        st->n_candidates = h_out[3 * nbt + 0];
+        st->n_recheck = h_out[3 * nbt + 1];
+        st->n_overflow = h_out[3 * nbt + 2];
+        st->cell_div = mst;
+    }
+    return GS_OK;
+}
+
+// ====================================================================== one-shot
+
+static gs_ctx* g_ctx = nullptr;
+
+static int oneshot_ctx(gs_ctx** c)
+{
+    if (!g_ctx) {
+        int rc = gs_create(&g_ctx, 0);
+        if (rc) return rc;
+    }
+    *c = g_ctx;
+    return GS_OK;
+}
+
+extern "C" int gs_pcf2(const double* xyz, const double* w, const int32_t* buf, int64_t n_data, int64_t n_rand,
+                       const gs_params* p, uint64_t* cnt, double* n2, double* n3, gs_stats* st)
+{
+    gs_ctx* c;
+    int rc = oneshot_ctx(&c);
+    if (rc) return rc;
+    if ((rc = gs_set_catalogue(c, xyz, w, buf, n_data, n_rand))) return rc;
+    return gs_count_2pcf(c, p, cnt, nullptr, n2, n3, st);
+}
+
+extern "C" int gs_pcf3(const double* xyz, const double* w, const int32_t* buf, int64_t n_data, int64_t n_rand,
+                       const gs_params* p, uint64_t* cnt, double* n2, double* n3, gs_stats* st)
+{
+    gs_ctx* c;
+    int rc = oneshot_ctx(&c);
+    if (rc) return rc;
+    if ((rc = gs_set_catalogue(c, xyz, w, buf, n_data, n_rand))) return rc;
+    return gs_count_3pcf(c, p, cnt, nullptr, n2, n3, st);
+}
+
+extern "C" int gs_pcf4(const double* xyz, const double* w, const int32_t* buf, int64_t n_data, int64_t n_rand,
+                       const gs_params* p, uint64_t* cnt, double* n2, double* n3, double* disc_n, double* disc_r,
+                       gs_stats* st)
+{
+    gs_ctx* c;
+    int rc = oneshot_ctx(&c);
+    if (rc) return rc;
+    if ((rc = gs_set_catalogue(c, xyz, w, buf, n_data, n_rand))) return rc;
+    return gs_count_4pcf(c, p, cnt, n2, n3, disc_n, disc_r, st);
+}
+
+// Internal accessors for the other translation units (3PCF/4PCF/adjoint).
+namespace gs {
+CtxView view(gs_ctx* c)
+{
+    CtxView v;
+    v.stream = c->stream;
+    v.ev = c->ev;
+    v.n_cu = c->n_cu;
+    v.n_data = c->n_data;
+    v.n_rand = c->n_rand;
+    v.has_w = c->has_w;
+    v.has_buf = c->has_buf;
+    v.wsum_d = c->wsum_d;
+    v.wsum_r = c->wsum_r;
+    v.wmax = c->wmax;
+    for (int k = 0; k < 6; ++k) v.bbox[k] = c->bbox[k];
+    v.g = c->g;
+    v.pw = c->pw.ptr;
+    v.flags = c->flags.ptr;
+    v.orig = c->orig.ptr;
+    v.cell_start = c->cell_start.ptr;
+    v.n_chunks = c->n_chunks;
+    v.extra = &c->extra;
+    return v;
+}
+int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out)
+{
+    double h = choose_h(c, rmax, m);
+    *h_out = h;
+    return build_grid(c, h);
+}
+int ctx_set_device(gs_ctx* c) { return set_device(c); }
+}  // namespace gs

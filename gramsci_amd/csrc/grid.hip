@@ -1,0 +1,210 @@
+// grid.hip -- GPU cell grid: bounding box, counting sort of points by cell,
+// cell start offsets.  Replaces the kd-tree build (kdtree2.F90:611-1012) and
+// its r-ball queries: a pair kernel reads each neighbour cell row as one
+// contiguous range of the sorted points.
+#include "gs_kernels.h"
+
+namespace gs {
+
+// ---------------------------------------------------------------- bounding box
+// Per-block min/max of the three coordinates; the host reduces the partials.
+__global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ xyz, int64_t n,
+                                              double* __restrict__ part /*[gridDim][6]*/)
+{
+    __shared__ double red[6][256];
+    double mn[3] = {1e308, 1e308, 1e308}, mx[3] = {-1e308, -1e308, -1e308};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            double v = xyz[3 * i + k];
+            mn[k] = fmin(mn[k], v);
+            mx[k] = fmax(mx[k], v);
+        }
+    }
+    for (int k = 0; k < 3; ++k) { red[k][threadIdx.x] = mn[k]; red[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+                red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------- cell of point
+__device__ __forceinline__ int cell_coord(double v, double lo, double inv_h, int dim)
+{
+    int c = (int)floor((v - lo) * inv_h);
+    return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
+}
+
+__global__ __launch_bounds__(256) void k_cell_count(const double* __restrict__ xyz, int64_t n, GridDesc g,
+                                                    int32_t* __restrict__ cell_of, int32_t* __restrict__ count)
+{
+    const double inv_h = 1.0 / g.h;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int cx = cell_coord(xyz[3 * i + 0], g.lo[0], inv_h, g.dim[0]);
+        int cy = cell_coord(xyz[3 * i + 1], g.lo[1], inv_h, g.dim[1]);
+        int cz = cell_coord(xyz[3 * i + 2], g.lo[2], inv_h, g.dim[2]);
+        int c = (cz * g.dim[1] + cy) * g.dim[0] + cx;
+        cell_of[i] = c;
+        atomicAdd(&count[c], 1);
+    }
+}
+
+// ---------------------------------------------------------------- exclusive scan
+// Three-phase scan of int32 counts: each block scans 1024 entries in LDS and
+// emits its total; the totals are scanned recursively; offsets are added back.
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(256) void k_scan_block(const int32_t* __restrict__ in, int64_t n,
+                                                    int32_t* __restrict__ out, int32_t* __restrict__ sums)
+{
+    __shared__ int32_t s[kScanBlock];
+    __shared__ int32_t wsum[4];
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock;
+    const int t = threadIdx.x;
+    int v[4];
+    int run = 0;
+    for (int k = 0; k < 4; ++k) {
+        int64_t i = base + t * 4 + k;
+        v[k] = i < n ? in[i] : 0;
+        run += v[k];
+    }
+    // inclusive scan of per-thread totals across the block (wave64 shuffles, then across 4 waves)
+    int x = run;
+    const int lane = t & 63, w = t >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int wpre = 0;
+    for (int k = 0; k < w; ++k) wpre += wsum[k];
+    int excl = wpre + x - run;
+    for (int k = 0; k < 4; ++k) {
+        s[t * 4 + k] = excl;
+        excl += v[k];
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+        int64_t i = base + t * 4 + k;
+        if (i < n) out[i] = s[t * 4 + k];
+    }
+    if (t == 255) sums[blockIdx.x] = excl;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int32_t* __restrict__ out, int64_t n, const int32_t* __restrict__ offs)
+{
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock;
+    const int32_t o = offs[blockIdx.x];
+    for (int k = threadIdx.x; k < kScanBlock; k += blockDim.x) {
+        int64_t i = base + k;
+        if (i < n) out[i] += o;
+    }
+}
+
+hipError_t exclusive_scan(const int32_t* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t st)
+{
+    // scratch must hold scan_scratch_size(n) int32
+    if (n <= 0) return hipSuccess;
+    int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    int32_t* sums = scratch;
+    int32_t* sums_scanned = scratch + nb;
+    hipLaunchKernelGGL(k_scan_block, dim3((unsigned)nb), dim3(256), 0, st, in, n, out, sums);
+    if (nb > 1) {
+        hipError_t e = exclusive_scan(sums, sums_scanned, nb, scratch + 2 * nb, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, st, out, n, sums_scanned);
+    }
+    return hipGetLastError();
+}
+
+int64_t scan_scratch_size(int64_t n)
+{
+    int64_t tot = 0;
+    while (n > 1) {
+        int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+        tot += 2 * nb;
+        n = nb;
+    }
+    return tot + 2;
+}
+
+// ---------------------------------------------------------------- scatter
+// Writes the points in cell order.  Order inside a cell follows the atomic
+// slot order; every count and fixed-point sum computed from the sorted arrays
+// is independent of it.
+__global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz, const double* __restrict__ w,
+                                                 const int32_t* __restrict__ buf, int64_t n, int64_t n_data,
+                                                 const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
+                                                 int32_t* __restrict__ fill, double4* __restrict__ pw,
+                                                 uint32_t* __restrict__ flags, int32_t* __restrict__ orig)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int c = cell_of[i];
+        int pos = start[c] + atomicAdd(&fill[c], 1);
+        pw[pos] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], w ? w[i] : 1.0);
+        uint32_t f = (i >= n_data ? 1u : 0u) | ((buf && buf[i] == 1) ? 2u : 0u);
+        flags[pos] = f;
+        orig[pos] = (int32_t)i;
+    }
+}
+
+hipError_t launch_bbox(const double* xyz, int64_t n, double* part, int nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bbox, dim3(nblk), dim3(256), 0, st, xyz, n, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_count(const double* xyz, int64_t n, const GridDesc& g, int32_t* cell_of, int32_t* count,
+                             hipStream_t st)
+{
+    int nblk = (int)((n + 255) / 256);
+    if (nblk > 65535) nblk = 65535;
+    if (nblk < 1) nblk = 1;
+    hipLaunchKernelGGL(k_cell_count, dim3(nblk), dim3(256), 0, st, xyz, n, g, cell_of, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
+                          const int32_t* cell_of, const int32_t* start, int32_t* fill, double4* pw,
+                          uint32_t* flags, int32_t* orig, hipStream_t st)
+{
+    int nblk = (int)((n + 255) / 256);
+    if (nblk > 65535) nblk = 65535;
+    if (nblk < 1) nblk = 1;
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, st, xyz, w, buf, n, n_data, cell_of, start, fill, pw,
+                       flags, orig);
+    return hipGetLastError();
+}
+
+}  // namespace gs
+
+namespace gs {
+
+// Number of work chunks (of kWG points) of each cell; entry ncell is 0 so that
+// an exclusive scan over ncell+1 entries ends with the total.
+__global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ start, int ncell,
+                                                      int32_t* __restrict__ nch)
+{
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += gridDim.x * blockDim.x) {
+        int v = 0;
+        if (c < ncell) v = (start[c + 1] - start[c] + kWG - 1) / kWG;
+        nch[c] = v;
+    }
+}
+
+hipError_t launch_chunk_counts(const int32_t* start, int ncell, int32_t* nch, hipStream_t st)
+{
+    int nblk = (ncell + 256) / 256;
+    if (nblk > 65535) nblk = 65535;
+    hipLaunchKernelGGL(k_chunk_counts, dim3(nblk), dim3(256), 0, st, start, ncell, nch);
+    return hipGetLastError();
+}
+
+}  // namespace gs

@@ -1,0 +1,63 @@
+// gs_api_internal.h -- host-side helpers shared by the translation units that
+// implement the C ABI (api.cpp, pcf3.cpp, pcf4.cpp, adjoint.cpp, batched.cpp).
+#pragma once
+
+#include <cstdarg>
+#include <string>
+#include <vector>
+
+#include "../../include/gramsci_hip.h"
+#include "gs_common.h"
+
+namespace gs {
+
+extern thread_local std::string g_err;
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+template <class T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    int reserve(size_t count);   // grows only; contents are not preserved
+    void release();
+};
+
+// Buffers owned by a context for the 3PCF/4PCF/adjoint/batched paths.
+struct GsExtra {
+    DevBuf<unsigned char> params;
+    DevBuf<unsigned long long> out;
+    DevBuf<int32_t> queue;
+    DevBuf<double> dbl;
+    DevBuf<int32_t> i32;
+};
+void extra_release(GsExtra& x);
+
+struct CtxView {
+    hipStream_t stream;
+    hipEvent_t* ev;
+    int n_cu;
+    int64_t n_data, n_rand;
+    bool has_w, has_buf;
+    long double wsum_d, wsum_r;
+    double wmax;
+    double bbox[6];
+    GridDesc g;
+    const double4* pw;
+    const uint32_t* flags;
+    const int32_t* orig;
+    const int32_t* cell_start;
+    int32_t n_chunks;
+    GsExtra* extra;
+};
+
+CtxView view(gs_ctx* c);
+int ctx_set_device(gs_ctx* c);
+int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
+int check_params(const gs_params* p, int mode);
+int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B);
+std::vector<Row> half_stencil(double h, double rmax, int m);
+long double fx_value(unsigned long long lo, unsigned long long hi, int shift);
+int default_cell_div(const gs_ctx* c, double rmax, int mode);
+
+}  // namespace gs

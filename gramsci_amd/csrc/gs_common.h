@@ -1,0 +1,50 @@
+// gs_common.h -- structures shared by the host side (api.cpp) and the HIP
+// kernels of the N-point correlation engine.  gfx950 (MI355X) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gs {
+
+constexpr int kMaxBins = 127;   // int8 bin storage in the reference (node_module.F90:7)
+constexpr int kWG = 256;        // work-group size of the counting kernels
+
+// Uniform cell grid over the catalogue's bounding box.  Points are stored
+// sorted by cell id = (cz*ny + cy)*nx + cx (x fastest), so one x-row of cells
+// is one contiguous range of points.
+struct GridDesc {
+    double lo[3];
+    double h;                // cell edge
+    int32_t dim[3];
+    int32_t ncell;
+    int32_t m;               // cells per rmax
+    int32_t pad;
+};
+
+// Binning constants.  t64 are the exact FP64 thresholds on s = |xj - xi|^2
+// (see gs_bin_thresholds): a pair is kept iff t64[0] < s < t64[nb+1]; its
+// 0-based bin is #{k in 1..nb : s >= t64[k]} (== nb means the reference's
+// out-of-range bin nbins+1).
+struct BinDesc {
+    int32_t nb, nm, log_bins, rsd;
+    double  t64[kMaxBins + 2];
+    double  mu_scale;        // 0.5*nmu (gramsci.F90:145)
+    // FP32 guard-band data
+    float   ok_lo[kMaxBins + 1];   // bin b is certain if ok_lo[b] <= s32 < ok_hi[b]
+    float   ok_hi[kMaxBins + 1];
+    float   rej_lo, rej_hi;        // s32 <= rej_lo or s32 >= rej_hi: certainly not kept
+    float   ua, ub;                // approximate bin: u = f(s)*ua + ub, f = sqrt (linear) or log2 (log)
+    float   ms32;                  // mu_scale in FP32
+    float   mu_ew, mu_em, mu_c;    // mu guard: dt = ms*(mu_ew/|sep| + mu_em/|mid| + mu_c)
+    // weighted fixed point: q = rint(w_i*w_j*... * 2^fx_shift)
+    int32_t fx_shift2, fx_shift3;
+};
+
+// Half-stencil rows of the pair kernel: cells (ax+dx, ay+dy, az+dz) with dx in [x0, x1].
+struct Row {
+    int8_t dy, dz, x0, x1;
+};
+constexpr int kMaxRows = 2048;
+
+}  // namespace gs

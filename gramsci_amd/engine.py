@@ -1,0 +1,136 @@
+"""Python host side of the engine: a thin object over the C ABI.
+
+    eng = Engine()                          # context on HIP device 0 (gfx950)
+    eng.set_catalogue(xyz, w, n_data)       # numpy (host) or torch.cuda tensors (device)
+    r = eng.pcf2(rmin, rmax, nbins, nmu=10, log_bins=False, precision="fp32")
+    r["cnt"], r["n2"], r["n3"], r["xi"], r["stats"]
+
+The counting path is always the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+PRECISION = {"fp64": L.GS_FP64, "fp32": L.GS_FP32_GUARD, "fp32_guard": L.GS_FP32_GUARD}
+
+
+def n3cfg(nbins: int) -> int:
+    return nbins * (nbins + 1) * (nbins + 2) // 6
+
+
+def n4cfg(nbins: int) -> int:
+    return nbins * (nbins + 1) * (nbins + 2) * (nbins + 3) // 24
+
+
+def _xi(n2, n3):
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return n2 / n3
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        self._ctx = ctypes.c_void_p()
+        L.check(self._lib.gs_create(ctypes.byref(self._ctx), int(device)))
+        self.n_data = 0
+        self.n_rand = 0
+
+    def close(self):
+        if self._ctx:
+            self._lib.gs_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ catalogue
+    def set_catalogue(self, xyz, w=None, n_data=None, buf=None):
+        """xyz (n,3) float64 with data first then randoms; w raw weights (n,) or None;
+        buf (n,) int32 with 1 = never an apex, or None.  numpy arrays are copied
+        from host memory; torch CUDA tensors are copied device to device."""
+        try:
+            import torch
+            is_dev = isinstance(xyz, torch.Tensor) and xyz.is_cuda
+        except ImportError:  # pragma: no cover
+            is_dev = False
+        n = int(xyz.shape[0])
+        nd = n if n_data is None else int(n_data)
+        if nd < 0 or nd > n:
+            raise ValueError("n_data out of range")
+        if is_dev:
+            import torch
+            xyz = xyz.to(torch.float64).contiguous()
+            w = None if w is None else w.to(torch.float64).contiguous()
+            buf = None if buf is None else buf.to(torch.int32).contiguous()
+            torch.cuda.current_stream().synchronize()
+            L.check(self._lib.gs_set_catalogue_device(
+                self._ctx, ctypes.c_void_p(xyz.data_ptr()),
+                None if w is None else ctypes.c_void_p(w.data_ptr()),
+                None if buf is None else ctypes.c_void_p(buf.data_ptr()), nd, n - nd))
+        else:
+            xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+            if xyz.ndim != 2 or xyz.shape[1] != 3:
+                raise ValueError("xyz must be (n, 3)")
+            w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+            buf = None if buf is None else np.ascontiguousarray(buf, dtype=np.int32)
+            if w is not None and w.shape != (n,):
+                raise ValueError("w must be (n,)")
+            if buf is not None and buf.shape != (n,):
+                raise ValueError("buf must be (n,)")
+            L.check(self._lib.gs_set_catalogue(self._ctx, L.ptr(xyz), L.ptr(w), L.ptr(buf, ctypes.c_int32),
+                                               nd, n - nd))
+        self.n_data, self.n_rand = nd, n - nd
+
+    # ------------------------------------------------------------ counts
+    @staticmethod
+    def params(rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
+        return L.make_params(rmin, rmax, nbins, nmu, log_bins, PRECISION[precision], cell_div)
+
+    def pcf2(self, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
+        p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
+        cnt = np.zeros((3, nbins, nmu), np.uint64)
+        wsum = np.zeros((3, nbins, nmu))
+        n2 = np.zeros((nbins, nmu))
+        n3 = np.zeros((nbins, nmu))
+        st = L.gs_stats()
+        L.check(self._lib.gs_count_2pcf(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(wsum),
+                                        L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
+        return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, xi=_xi(n2, n3), stats=st.as_dict())
+
+    def pcf3(self, rmin, rmax, nbins, log_bins=False, precision="fp32", cell_div=0):
+        p = self.params(rmin, rmax, nbins, 1, log_bins, precision, cell_div)
+        nc = n3cfg(nbins)
+        cnt = np.zeros((nc, 4), np.uint64)
+        wsum = np.zeros((nc, 4))
+        n2 = np.zeros(nc)
+        n3 = np.zeros(nc)
+        st = L.gs_stats()
+        L.check(self._lib.gs_count_3pcf(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(wsum),
+                                        L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
+        return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, zeta=_xi(n2, n3), stats=st.as_dict())
+
+    def pcf4(self, rmin, rmax, nbins, log_bins=False, precision="fp64", cell_div=0):
+        p = self.params(rmin, rmax, nbins, 1, log_bins, precision, cell_div)
+        nc = n4cfg(nbins)
+        cnt = np.zeros((nc, nbins, nbins, 16), np.uint64)
+        n2 = np.zeros((nc, nbins, nbins))
+        n3 = np.zeros((nc, nbins, nbins))
+        dn = np.zeros((6, nc, nbins, nbins))
+        dr = np.zeros((6, nc, nbins, nbins))
+        st = L.gs_stats()
+        L.check(self._lib.gs_count_4pcf(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(n2),
+                                        L.ptr(n3), L.ptr(dn), L.ptr(dr), ctypes.byref(st)))
+        return dict(cnt=cnt, n2=n2, n3=n3, disc_n=dn, disc_r=dr, eta=_xi(n2, n3), stats=st.as_dict())

@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -66,6 +67,7 @@ template struct DevBuf<double>;
 template struct DevBuf<int32_t>;
 template struct DevBuf<uint32_t>;
 template struct DevBuf<double4>;
+template struct DevBuf<float4>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<unsigned char>;
 
@@ -260,6 +262,27 @@ std::vector<Row> half_stencil(double h, double rmax, int m)
     return rows;
 }
 
+// Full stencil (both half spaces) as x-rows, for apex-centred kernels.
+std::vector<Row> full_stencil(double h, double rmax, int m)
+{
+    std::vector<Row> half = half_stencil(h, rmax, m);
+    std::vector<Row> rows;
+    for (const Row& r : half) {
+        if (r.dy == 0 && r.dz == 0) {
+            Row c = r;
+            c.x0 = (int8_t)(-r.x1);
+            rows.push_back(c);
+        } else {
+            rows.push_back(r);
+            Row n = r;
+            n.dy = (int8_t)(-r.dy);
+            n.dz = (int8_t)(-r.dz);
+            rows.push_back(n);
+        }
+    }
+    return rows;
+}
+
 // 128-bit fixed point (lo, hi) scaled by 2^-shift -> long double
 long double fx_value(unsigned long long lo, unsigned long long hi, int shift)
 {
@@ -290,6 +313,7 @@ struct gs_ctx {
     double grid_h = 0.0;
     DevBuf<int32_t> cell_of, cell_start, fill, chunk_nc, chunk_off, scan_tmp;
     DevBuf<double4> pw;
+    DevBuf<float4> pl;
     DevBuf<uint32_t> flags;
     DevBuf<int32_t> orig;
     int32_t n_chunks = 0;
@@ -347,7 +371,8 @@ extern "C" int gs_destroy(gs_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     c->xyz.release(); c->w.release(); c->buf.release();
     c->cell_of.release(); c->cell_start.release(); c->fill.release(); c->chunk_nc.release();
-    c->chunk_off.release(); c->scan_tmp.release(); c->pw.release(); c->flags.release(); c->orig.release();
+    c->chunk_off.release(); c->scan_tmp.release(); c->pw.release(); c->pl.release(); c->flags.release();
+    c->orig.release();
     c->params.release(); c->out.release(); c->queue.release(); c->part.release();
     extra_release(c->extra);
     for (auto& ev : c->ev)
@@ -471,14 +496,17 @@ namespace gs {
 static double choose_h(const gs_ctx* c, double rmax, int m)
 {
     const int64_t kMaxCells = 1 << 24;
-    double h = rmax / m;
+    // round up to 11 significant bits: then k*h is exact in FP32 for |k| < 2^13
+    int ex;
+    double fr = std::frexp(rmax / m, &ex);
+    double h = std::ldexp(std::ceil(std::ldexp(fr, 11)), ex - 11);
     double ext = 0;
     for (int k = 0; k < 3; ++k) ext = std::max(ext, c->bbox[3 + k] - c->bbox[k]);
     for (;;) {
         int64_t nc = 1;
         for (int k = 0; k < 3; ++k) nc *= (int64_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
         if (nc <= kMaxCells) break;
-        h *= 1.25;
+        h = std::ldexp(std::ceil(std::ldexp(std::frexp(h * 1.25, &ex), 11)), ex - 11);
     }
     (void)ext;
     return h;
@@ -498,10 +526,11 @@ int build_grid(gs_ctx* c, double h)
     }
     if (ncell > (1 << 26)) return fail(GS_ELIMIT, "grid too large (%lld cells)", (long long)ncell);
     g.ncell = (int32_t)ncell;
+    g.h32 = (float)h;
     int rc;
     if ((rc = c->cell_of.reserve(n)) || (rc = c->cell_start.reserve(ncell + 1)) || (rc = c->fill.reserve(ncell + 1)) ||
         (rc = c->chunk_nc.reserve(ncell + 1)) || (rc = c->chunk_off.reserve(ncell + 1)) ||
-        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) ||
+        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) || (rc = c->pl.reserve(n)) ||
         (rc = c->flags.reserve(n)) || (rc = c->orig.reserve(n)))
         return rc;
     hipError_t e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
@@ -510,13 +539,14 @@ int build_grid(gs_ctx* c, double h)
     if (e == hipSuccess) e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
     if (e == hipSuccess)
         e = launch_scatter(c->xyz.ptr, c->has_w ? c->w.ptr : nullptr, c->has_buf ? c->buf.ptr : nullptr, n, c->n_data,
-                           c->cell_of.ptr, c->cell_start.ptr, c->fill.ptr, c->pw.ptr, c->flags.ptr, c->orig.ptr,
-                           c->stream);
-    if (e == hipSuccess) e = launch_chunk_counts(c->cell_start.ptr, (int)ncell, c->chunk_nc.ptr, c->stream);
-    if (e == hipSuccess) e = exclusive_scan(c->chunk_nc.ptr, c->chunk_off.ptr, ncell + 1, c->scan_tmp.ptr, c->stream);
+                           g, c->cell_of.ptr, c->cell_start.ptr, c->fill.ptr, c->pw.ptr, c->pl.ptr, c->flags.ptr,
+                           c->orig.ptr, c->stream);
+    const int nrow = g.dim[1] * g.dim[2];
+    if (e == hipSuccess) e = launch_row_chunks(c->cell_start.ptr, g.dim[0], nrow, c->chunk_nc.ptr, c->stream);
+    if (e == hipSuccess) e = exclusive_scan(c->chunk_nc.ptr, c->chunk_off.ptr, nrow + 1, c->scan_tmp.ptr, c->stream);
     int32_t nch = 0;
     if (e == hipSuccess)
-        e = hipMemcpyAsync(&nch, c->chunk_off.ptr + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(&nch, c->chunk_off.ptr + nrow, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e, "grid build");
     c->g = g;
@@ -575,6 +605,10 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     int reps = 1;
     while (reps < 32 && (size_t)nbt * reps * 2 * 4 <= 32 * 1024) reps *= 2;
     while (reps > 1 && pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted) > 64 * 1024) reps /= 2;
+    if (const char* ev = getenv("GS_PAIR_REPS")) {   // tuning override (power of two)
+        int v = atoi(ev);
+        if (v >= 1 && v <= 32 && (v & (v - 1)) == 0) reps = v;
+    }
     if (pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted) > 64 * 1024)
         return fail(GS_ELIMIT, "histogram of %d bins does not fit in LDS", nbt);
 
@@ -593,9 +627,10 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
 
     PairArgs a{};
     a.pw = c->pw.ptr;
+    a.pl = c->pl.ptr;
     a.flags = c->flags.ptr;
     a.cell_start = c->cell_start.ptr;
-    a.chunk_off = c->chunk_off.ptr;
+    a.row_chunk = c->chunk_off.ptr;
     a.rows = dRows;
     a.bins = (const BinDesc*)dB;
     a.queue = c->queue.ptr;
@@ -607,6 +642,7 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     a.nrows = (int)rows.size();
     a.n_chunks = c->n_chunks;
     a.reps = reps;
+    a.mirror = (rsd && !c->has_buf) ? 1 : 0;
     const int occ = pair2_occupancy(nb, nm, reps, fp64, rsd, weighted);
     int nblocks = c->n_cu * occ;
     if (nblocks > c->n_chunks) nblocks = std::max(c->n_chunks, 1);
@@ -735,6 +771,12 @@ int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out)
     double h = choose_h(c, rmax, m);
     *h_out = h;
     return build_grid(c, h);
+}
+double ctx_xmax(gs_ctx* c)
+{
+    double xmax = 0;
+    for (int k = 0; k < 6; ++k) xmax = std::max(xmax, std::fabs(c->bbox[k]));
+    return xmax;
 }
 int ctx_set_device(gs_ctx* c) { return set_device(c); }
 }  // namespace gs

@@ -140,19 +140,46 @@ int64_t scan_scratch_size(int64_t n)
 // slot order; every count and fixed-point sum computed from the sorted arrays
 // is independent of it.
 __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz, const double* __restrict__ w,
-                                                 const int32_t* __restrict__ buf, int64_t n, int64_t n_data,
+                                                 const int32_t* __restrict__ buf, int64_t n, int64_t n_data, GridDesc g,
                                                  const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
                                                  int32_t* __restrict__ fill, double4* __restrict__ pw,
-                                                 uint32_t* __restrict__ flags, int32_t* __restrict__ orig)
+                                                 float4* __restrict__ pl, uint32_t* __restrict__ flags,
+                                                 int32_t* __restrict__ orig)
 {
+    const int nx = g.dim[0], ny = g.dim[1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int c = cell_of[i];
-        int pos = start[c] + atomicAdd(&fill[c], 1);
-        pw[pos] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], w ? w[i] : 1.0);
-        uint32_t f = (i >= n_data ? 1u : 0u) | ((buf && buf[i] == 1) ? 2u : 0u);
+        const int c = cell_of[i];
+        const int pos = start[c] + atomicAdd(&fill[c], 1);
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        pw[pos] = make_double4(x, y, z, w ? w[i] : 1.0);
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        // cell-local FP32 coordinates; .w carries the cell's x index (exact in FP32)
+        pl[pos] = make_float4((float)(x - (g.lo[0] + cx * g.h)), (float)(y - (g.lo[1] + cy * g.h)),
+                              (float)(z - (g.lo[2] + cz * g.h)), (float)cx);
+        const uint32_t f = (i >= n_data ? 1u : 0u) | ((buf && buf[i] == 1) ? 2u : 0u);
         flags[pos] = f;
         orig[pos] = (int32_t)i;
     }
+}
+
+// Work chunks of the pair kernel: kChunk consecutive points of one x-row of
+// cells (fixed cy, cz).  nch[r] = chunks of row r; nch[nrow] = 0.
+__global__ __launch_bounds__(256) void k_row_chunks(const int32_t* __restrict__ start, int nx, int nrow,
+                                                    int32_t* __restrict__ nch)
+{
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r <= nrow; r += gridDim.x * blockDim.x) {
+        int v = 0;
+        if (r < nrow) v = (start[(r + 1) * nx] - start[r * nx] + kChunk - 1) / kChunk;
+        nch[r] = v;
+    }
+}
+
+hipError_t launch_row_chunks(const int32_t* start, int nx, int nrow, int32_t* nch, hipStream_t st)
+{
+    int nblk = (nrow + 256) / 256;
+    if (nblk > 65535) nblk = 65535;
+    hipLaunchKernelGGL(k_row_chunks, dim3(nblk), dim3(256), 0, st, start, nx, nrow, nch);
+    return hipGetLastError();
 }
 
 hipError_t launch_bbox(const double* xyz, int64_t n, double* part, int nblk, hipStream_t st)
@@ -172,39 +199,16 @@ hipError_t launch_cell_count(const double* xyz, int64_t n, const GridDesc& g, in
 }
 
 hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
-                          const int32_t* cell_of, const int32_t* start, int32_t* fill, double4* pw,
-                          uint32_t* flags, int32_t* orig, hipStream_t st)
+                          const GridDesc& g, const int32_t* cell_of, const int32_t* start, int32_t* fill,
+                          double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
 {
     int nblk = (int)((n + 255) / 256);
     if (nblk > 65535) nblk = 65535;
     if (nblk < 1) nblk = 1;
-    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, st, xyz, w, buf, n, n_data, cell_of, start, fill, pw,
-                       flags, orig);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, st, xyz, w, buf, n, n_data, g, cell_of, start, fill, pw,
+                       pl, flags, orig);
     return hipGetLastError();
 }
 
 }  // namespace gs
 
-namespace gs {
-
-// Number of work chunks (of kWG points) of each cell; entry ncell is 0 so that
-// an exclusive scan over ncell+1 entries ends with the total.
-__global__ __launch_bounds__(256) void k_chunk_counts(const int32_t* __restrict__ start, int ncell,
-                                                      int32_t* __restrict__ nch)
-{
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += gridDim.x * blockDim.x) {
-        int v = 0;
-        if (c < ncell) v = (start[c + 1] - start[c] + kWG - 1) / kWG;
-        nch[c] = v;
-    }
-}
-
-hipError_t launch_chunk_counts(const int32_t* start, int ncell, int32_t* nch, hipStream_t st)
-{
-    int nblk = (ncell + 256) / 256;
-    if (nblk > 65535) nblk = 65535;
-    hipLaunchKernelGGL(k_chunk_counts, dim3(nblk), dim3(256), 0, st, start, ncell, nch);
-    return hipGetLastError();
-}
-
-}  // namespace gs

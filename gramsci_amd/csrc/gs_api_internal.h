@@ -57,6 +57,8 @@ int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
 int check_params(const gs_params* p, int mode);
 int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B);
 std::vector<Row> half_stencil(double h, double rmax, int m);
+std::vector<Row> full_stencil(double h, double rmax, int m);
+double ctx_xmax(gs_ctx* c);
 long double fx_value(unsigned long long lo, unsigned long long hi, int shift);
 int default_cell_div(const gs_ctx* c, double rmax, int mode);
 

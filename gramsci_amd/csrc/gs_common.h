@@ -9,17 +9,19 @@ namespace gs {
 
 constexpr int kMaxBins = 127;   // int8 bin storage in the reference (node_module.F90:7)
 constexpr int kWG = 256;        // work-group size of the counting kernels
+constexpr int kWave = 64;       // CDNA wavefront
+constexpr int kChunk = 64;      // points per work item of the pair kernel (one per lane)
 
 // Uniform cell grid over the catalogue's bounding box.  Points are stored
 // sorted by cell id = (cz*ny + cy)*nx + cx (x fastest), so one x-row of cells
 // is one contiguous range of points.
 struct GridDesc {
     double lo[3];
-    double h;                // cell edge
+    double h;                // cell edge (a float with <= 11 significant bits, so k*h is exact in FP32)
     int32_t dim[3];
     int32_t ncell;
     int32_t m;               // cells per rmax
-    int32_t pad;
+    float h32;               // h as float (exact)
 };
 
 // Binning constants.  t64 are the exact FP64 thresholds on s = |xj - xi|^2

@@ -12,16 +12,17 @@ hipError_t launch_cell_count(const double* xyz, int64_t n, const GridDesc& g, in
 hipError_t exclusive_scan(const int32_t* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t st);
 int64_t scan_scratch_size(int64_t n);
 hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
-                          const int32_t* cell_of, const int32_t* start, int32_t* fill, double4* pw,
-                          uint32_t* flags, int32_t* orig, hipStream_t st);
-hipError_t launch_chunk_counts(const int32_t* start, int ncell, int32_t* nch, hipStream_t st);
+                          const GridDesc& g, const int32_t* cell_of, const int32_t* start, int32_t* fill,
+                          double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
+hipError_t launch_row_chunks(const int32_t* start, int nx, int nrow, int32_t* nch, hipStream_t st);
 
 // Arguments of the persistent 2PCF pair kernel (pair2.hip).
 struct PairArgs {
     const double4* pw;          // sorted points: x, y, z, raw weight
+    const float4* pl;           // sorted: cell-local FP32 x, y, z and the cell x index
     const uint32_t* flags;      // sorted: bit0 random, bit1 never-apex
     const int32_t* cell_start;  // ncell+1
-    const int32_t* chunk_off;   // ncell+1: first work chunk of each cell (chunks of kWG points)
+    const int32_t* row_chunk;   // nrow+1: first work chunk of each x-row of cells
     const Row* rows;            // half stencil
     const BinDesc* bins;        // device copy
     int32_t* queue;             // 8 work counters (one per XCD segment), zeroed before launch
@@ -31,11 +32,60 @@ struct PairArgs {
     GridDesc g;
     int32_t nrows;
     int32_t n_chunks;
-    int32_t reps;               // LDS histogram replicas (power of two <= 64)
-    int32_t pad;
-    float oshift;               // unused placeholder (keeps the struct 8-byte aligned)
-    float pad2;
+    int32_t reps;               // LDS histogram replicas (power of two <= 32)
+    int32_t mirror;             // (r, mu) with no buffer points: one update per unordered pair
 };
+
+// Arguments of the persistent 3PCF triplet kernel (triplet3.hip).
+struct TripArgs {
+    const double4* pw;
+    const uint32_t* flags;
+    const int32_t* cell_start;
+    const Row* rows;            // full stencil
+    const BinDesc* bins;
+    const uint16_t* cfg_tab;    // [nb][nb][nb] -> 0-based configuration
+    int32_t* queue;             // 8 work counters
+    int32_t* err;               // set to 1 if an apex has more than kmax neighbours
+    unsigned long long* out_cnt;    // [ncfg][4]
+    unsigned long long* out_w;      // [ncfg][4][2]
+    unsigned long long* out_stats;  // candidates, rechecks, overflow, units, binned
+    GridDesc g;
+    int32_t nrows;
+    int32_t n_items;            // apexes (all points; buffer points are skipped)
+    int32_t ncfg;
+    int32_t reps;
+    int32_t kmax;
+    int32_t pad;
+};
+size_t triplet3_lds_bytes(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted);
+hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);
+int triplet3_occupancy(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted);
+
+// Arguments of the persistent 4PCF quad kernel (quad4.hip).
+struct QuadArgs {
+    const double4* pw;
+    const uint32_t* flags;
+    const int32_t* orig;        // original input index of each sorted point
+    const int32_t* cell_start;
+    const Row* rows;            // full stencil
+    const BinDesc* bins;
+    const uint16_t* cfg_tab;    // [nb]^4 -> 1-based perimeter configuration, 0 = not sorted
+    int32_t* queue;
+    int32_t* err;
+    unsigned long long* out_cnt;    // [ncell][16]
+    double* out_w;              // weighted: [ncell][16] quad products, [6][ncell][4] disc pair products
+    unsigned long long* out_stats;
+    GridDesc g;
+    int32_t nrows;
+    int32_t n_items;
+    int32_t ncell;
+    int32_t kmax;
+    int32_t lds_hist;           // counts in LDS (else global atomics)
+    int32_t pad;
+};
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist);
+hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);
+int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool weighted);
 
 // Dynamic LDS bytes of the pair kernel for these parameters.
 size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted);

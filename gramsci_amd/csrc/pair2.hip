@@ -6,26 +6,39 @@
 // only if its apex is not a buffer point, :363), with mu bins m and nmu-1-m
 // (theta(j->i) = -theta(i->j) exactly).
 //
-// Work decomposition: a work item is a chunk of <= 256 points of one home cell
-// A (one point per lane).  The item walks the half stencil of A as x-rows of
-// cells; an x-row is one contiguous range of the cell-sorted points, staged
-// through LDS in tiles of 256 and broadcast to every lane.  Work-groups are
-// persistent and pull items from 8 queues, one per XCD, each covering a
-// contiguous z-slab of the grid (so an XCD's L2 sees one slab), then steal.
+// Work decomposition.  Points are sorted by cell with x fastest, so an x-row
+// of cells (fixed cy, cz) is one contiguous range.  A work item is a chunk of
+// 64 consecutive points of one x-row, one point per lane of a wavefront.  The
+// wave walks the half stencil of x-rows (dy, dz); for each it streams the
+// contiguous range of cells [cx_first + x0, cx_last + x1] through a
+// wave-private LDS tile of 64 points that every lane reads by broadcast.
+// Waves are persistent and independent (no work-group barriers in the main
+// loop); they pull items from 8 queues, one per XCD, each covering a
+// contiguous z-slab of the grid so that an XCD's L2 sees one slab, and steal
+// from the other queues when their own is empty.
 //
-// Precision variants:
+// Precision variants (identical counts):
 //   FP64        s = |xj-xi|^2 in FP64 exactly as the reference; bins from the
 //               exact thresholds; mu with IEEE sqrt/div.
-//   FP32 guard  coordinates relative to A's cell origin in FP32; a pair is
-//               binned directly only when s32 lies inside the bin's certain
-//               interval [ok_lo, ok_hi) and mu is farther than a rigorous
-//               error bound from a mu edge; otherwise it is re-evaluated with
-//               the FP64 path.  Both variants produce identical counts.
+//   FP32 guard  d = (xj_l - xi_l) + (cx_j - cx_i) * h from cell-local FP32
+//               coordinates (h has <= 11 significant bits, so the cell offset
+//               is exact).  A pair is binned directly only when s32 lies in the
+//               bin's certain interval [ok_lo, ok_hi) and its mu is farther
+//               than a rigorous error bound from a mu edge; otherwise it is
+//               re-evaluated with the FP64 path.
 //
-// Histograms live in LDS: u32 counts replicated `reps` times (replica =
-// lane % reps) to spread same-bin atomics over banks, and one 128-bit fixed
-// point weighted sum per bin.  They are flushed to global u64 / 128-bit
-// accumulators when the work-group ends (or before a u32 could overflow).
+// (r, mu) without buffer points: every pair is counted in both directions and
+// theta(j->i) = -theta(i->j) exactly, so one update at mu bin m stands for
+// both ordered pairs (m and nm-1-m) and the drain adds it to both bins.  Only
+// pairs exactly on a mu edge (where the two directions are not mirror images)
+// are added explicitly.
+//
+// Histograms live in LDS, shared by the work-group's waves: u32 counts
+// replicated `reps` times (replica = lane % reps) to spread same-bin atomics,
+// and per bin two u64 sums of the weight product's fixed-point value split at
+// bit 32 (no carries, so no returning atomics).  A wave that has issued 2^29
+// increments drains the histogram with atomic exchanges into the global
+// 128-bit accumulators; the rest is drained when the work-group ends.
 #include "gs_kernels.h"
 
 namespace gs {
@@ -36,12 +49,6 @@ struct Stats {
     unsigned long long cand, rechk, over;
 };
 
-__device__ __forceinline__ void lds_add128(unsigned long long* slot, unsigned long long q)
-{
-    unsigned long long old = atomicAdd(slot, q);
-    if (old + q < old) atomicAdd(slot + 1, 1ull);
-}
-
 __device__ __forceinline__ void glb_add128(unsigned long long* slot, unsigned long long lo, unsigned long long hi)
 {
     unsigned long long old = atomicAdd(slot, lo);
@@ -50,9 +57,9 @@ __device__ __forceinline__ void glb_add128(unsigned long long* slot, unsigned lo
 }
 
 // Reference arithmetic in FP64, no contraction: s (kdtree2.F90:1689, gramsci.F90:1532).
-__device__ __forceinline__ double exact_s(const double4& a, const double4& b)
+__device__ __forceinline__ double exact_s(double ax, double ay, double az, double bx, double by, double bz)
 {
-    double dx = __dsub_rn(b.x, a.x), dy = __dsub_rn(b.y, a.y), dz = __dsub_rn(b.z, a.z);
+    double dx = __dsub_rn(bx, ax), dy = __dsub_rn(by, ay), dz = __dsub_rn(bz, az);
     return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
 }
 
@@ -67,9 +74,8 @@ __device__ __forceinline__ int exact_bin(double s, const double* t, int nb)
     return lo;
 }
 
-// mu bin of the ordered pair apex a -> neighbour b (gramsci.F90:320-328), 0-based,
-// and of the reverse pair b -> a.  The reference stores floor(.)+1 in an int8
-// before clamping; that wrap is reproduced.
+// mu bin from t = (theta+1)*mu_scale (gramsci.F90:326-328), 0-based.  The
+// reference stores floor(t)+1 in an int8 before clamping; that wrap is kept.
 __device__ __forceinline__ int mu_from_t(double t, int nm)
 {
     int v = (int)floor(t) + 1;
@@ -78,12 +84,14 @@ __device__ __forceinline__ int mu_from_t(double t, int nm)
     return v - 1;
 }
 
-__device__ __forceinline__ void exact_mu(const double4& a, const double4& b, double ms, int nm, int& m, int& mr)
+// mu bins of apex a -> neighbour b and of b -> a (gramsci.F90:320-325).
+__device__ __forceinline__ void exact_mu(double ax, double ay, double az, double bx, double by, double bz, double ms,
+                                         int nm, int& m, int& mr)
 {
-    double m1 = __dmul_rn(0.5, __dadd_rn(a.x, b.x));
-    double m2 = __dmul_rn(0.5, __dadd_rn(a.y, b.y));
-    double m3 = __dmul_rn(0.5, __dadd_rn(a.z, b.z));
-    double w1 = __dsub_rn(a.x, b.x), w2 = __dsub_rn(a.y, b.y), w3 = __dsub_rn(a.z, b.z);
+    double m1 = __dmul_rn(0.5, __dadd_rn(ax, bx));
+    double m2 = __dmul_rn(0.5, __dadd_rn(ay, by));
+    double m3 = __dmul_rn(0.5, __dadd_rn(az, bz));
+    double w1 = __dsub_rn(ax, bx), w2 = __dsub_rn(ay, by), w3 = __dsub_rn(az, bz);
     double th = __dadd_rn(__dadd_rn(__dmul_rn(m1, w1), __dmul_rn(m2, w2)), __dmul_rn(m3, w3));
     double mm = __dadd_rn(__dadd_rn(__dmul_rn(m1, m1), __dmul_rn(m2, m2)), __dmul_rn(m3, m3));
     double ww = __dadd_rn(__dadd_rn(__dmul_rn(w1, w1), __dmul_rn(w2, w2)), __dmul_rn(w3, w3));
@@ -93,64 +101,114 @@ __device__ __forceinline__ void exact_mu(const double4& a, const double4& b, dou
     mr = mu_from_t(__dmul_rn(__dadd_rn(-th, 1.0), ms), nm);
 }
 
-__device__ __forceinline__ unsigned long long quantise(double p, int shift)
+// floor(x) split at bit 32, x = w_i * (w_j * 2^shift) < 2^61 (the scale is
+// folded into the staged w_j; scaling by 2^shift is exact):
+// q = hi * 2^32 + lo, lo < 2^32, hi < 2^29.  Truncation moves each term by
+// less than 2^-shift, at most 2^-61 of the largest product.
+__device__ __forceinline__ void quantise(double x, uint32_t& lo, uint32_t& hi)
 {
-    return __double2ull_rn(ldexp(p, shift));
+    const double h = floor(x * 0x1p-32);
+    hi = (uint32_t)h;
+    lo = (uint32_t)fma(-h, 0x1p32, x);
 }
 
 template <bool RSD, bool W>
 __device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw, int reps, int rep, int nb, int nm,
-                                           int cls, int b, int m, int mr, int ci, int cj, double p, int shift)
+                                           bool mirror, int cls, int b, int m, int mr, int ci, int cj, double p,
+                                           int shift, unsigned long long* out_cnt, unsigned long long* out_w)
 {
     const int base = (cls * nb + b) * nm;
+    uint32_t qlo = 0, qhi = 0;
+    if constexpr (W) quantise(p, qlo, qhi);
     if constexpr (!RSD) {
         const int c = ci + cj;
         if (c) {
             atomicAdd(&hc[base * reps + rep], (uint32_t)c);
-            if constexpr (W) lds_add128(&hw[2 * base], quantise(p, shift) * (unsigned long long)c);
+            if constexpr (W) {
+                atomicAdd(&hw[2 * base], (unsigned long long)qlo * c);
+                atomicAdd(&hw[2 * base + 1], (unsigned long long)qhi * c);
+            }
+        }
+    } else if (mirror && mr == nm - 1 - m) {
+        // mirror-implied histogram: this update stands for i->j at m and j->i at nm-1-m
+        atomicAdd(&hc[(base + m) * reps + rep], 1u);
+        if constexpr (W) {
+            atomicAdd(&hw[2 * (base + m)], (unsigned long long)qlo);
+            atomicAdd(&hw[2 * (base + m) + 1], (unsigned long long)qhi);
+        }
+    } else if (mirror) {
+        // a pair exactly on a mu edge: both ordered pairs explicitly, straight to global memory (rare)
+        const unsigned long long L = ((unsigned long long)qhi << 32) + qlo;
+        const unsigned long long H = (unsigned long long)qhi >> 32;
+        atomicAdd(&out_cnt[base + m], 1ull);
+        atomicAdd(&out_cnt[base + mr], 1ull);
+        if constexpr (W) {
+            glb_add128(&out_w[2 * (base + m)], L, H);
+            glb_add128(&out_w[2 * (base + mr)], L, H);
         }
     } else {
-        unsigned long long q = 0;
-        if constexpr (W) q = quantise(p, shift);
         if (ci) {
             atomicAdd(&hc[(base + m) * reps + rep], 1u);
-            if constexpr (W) lds_add128(&hw[2 * (base + m)], q);
+            if constexpr (W) {
+                atomicAdd(&hw[2 * (base + m)], (unsigned long long)qlo);
+                atomicAdd(&hw[2 * (base + m) + 1], (unsigned long long)qhi);
+            }
         }
         if (cj) {
             atomicAdd(&hc[(base + mr) * reps + rep], 1u);
-            if constexpr (W) lds_add128(&hw[2 * (base + mr)], q);
+            if constexpr (W) {
+                atomicAdd(&hw[2 * (base + mr)], (unsigned long long)qlo);
+                atomicAdd(&hw[2 * (base + mr) + 1], (unsigned long long)qhi);
+            }
         }
     }
 }
 
-__device__ void flush(uint32_t* hc, unsigned long long* hw, int nbt, int reps, bool weighted,
-                      unsigned long long* out_cnt, unsigned long long* out_w)
+// Drains the LDS histogram into the global accumulators with atomic
+// exchanges, so any wave may do it while the others keep adding.  Called by
+// all lanes of one wave (stride 64) or of the whole work-group (stride 256).
+// With `mirror`, entry (cls, b, m) also stands for (cls, b, nm-1-m).
+__device__ void drain(uint32_t* hc, unsigned long long* hw, int nbt, int nm, int reps, bool weighted, bool mirror,
+                      int t0, int stride, unsigned long long* out_cnt, unsigned long long* out_w)
 {
-    for (int k = threadIdx.x; k < nbt; k += blockDim.x) {
+    for (int k = t0; k < nbt; k += stride) {
+        const int km = mirror ? (k - k % nm) + (nm - 1 - k % nm) : k;
         unsigned long long s = 0;
-        for (int r = 0; r < reps; ++r) {
-            s += hc[k * reps + r];
-            hc[k * reps + r] = 0;
+        for (int r = 0; r < reps; ++r) s += atomicExch(&hc[k * reps + r], 0u);
+        if (s) {
+            atomicAdd(&out_cnt[k], s);
+            if (mirror) atomicAdd(&out_cnt[km], s);
         }
-        if (s) atomicAdd(&out_cnt[k], s);
         if (weighted) {
-            unsigned long long lo = hw[2 * k], hi = hw[2 * k + 1];
-            if (lo | hi) glb_add128(&out_w[2 * k], lo, hi);
-            hw[2 * k] = 0;
-            hw[2 * k + 1] = 0;
+            const unsigned long long lo = atomicExch(&hw[2 * k], 0ull);
+            const unsigned long long hi = atomicExch(&hw[2 * k + 1], 0ull);
+            if (lo | hi) {
+                // value = hi * 2^32 + lo as a 128-bit number
+                const unsigned long long L = (hi << 32) + lo;
+                const unsigned long long H = (hi >> 32) + (L < lo ? 1ull : 0ull);
+                glb_add128(&out_w[2 * k], L, H);
+                if (mirror) glb_add128(&out_w[2 * km], L, H);
+            }
         }
     }
 }
 
-__device__ __forceinline__ int find_cell(const int32_t* chunk_off, int ncell, int chunk)
+__device__ __forceinline__ int find_row(const int32_t* row_chunk, int nrow, int chunk)
 {
-    // largest A with chunk_off[A] <= chunk
-    int lo = 0, hi = ncell - 1;
+    // largest r with row_chunk[r] <= chunk
+    int lo = 0, hi = nrow - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (chunk_off[mid] <= chunk) lo = mid; else hi = mid - 1;
+        if (row_chunk[mid] <= chunk) lo = mid; else hi = mid - 1;
     }
     return lo;
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 }  // namespace
@@ -160,11 +218,12 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted)
     const size_t nbt = (size_t)3 * nb * nm;
     size_t b = 0;
     b += ((nbt * reps * 4 + 15) / 16) * 16;                  // counts
-    if (weighted) b += nbt * 16;                              // 128-bit sums
+    if (weighted) b += nbt * 16;                              // split fixed-point sums
     b += ((size_t)(kMaxBins + 1) * 8);                        // ok_lo/ok_hi as float2
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
-    if (fp64) b += (size_t)kWG * (24 + 8 + 4 + 4);            // x, y, z, w, flags, pad
-    else b += (size_t)kWG * (16 + 8);                         // float4, w
+    const int waves = kWG / kWave;
+    if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // x, y, z, w, flags, pad
+    else b += (size_t)waves * kWave * (16 + 8 + 4 + 4);       // float4, w, flags, pad
     return b;
 }
 
@@ -178,7 +237,10 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     const int nbt = 3 * nb * nm;
     const int reps = a.reps;
     const int tid = threadIdx.x;
-    const int rep = tid & (reps - 1);
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
+    const int rep = lane & (reps - 1);
+    const bool mirror = RSD && a.mirror;
 
     unsigned char* p = smem;
     uint32_t* hc = (uint32_t*)p;
@@ -189,20 +251,21 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
-    // tiles
-    float4* tile32 = (float4*)p;
-    double* tx = (double*)p;
-    double* ty = tx + kWG;
-    double* tz = ty + kWG;
-    double* tw = FP64 ? (tz + kWG) : (double*)(tile32 + kWG);
-    uint32_t* tf = (uint32_t*)(tw + kWG);
+    // wave-private tiles
+    constexpr int kTileBytes = FP64 ? kWave * 40 : kWave * 32;
+    unsigned char* tb = p + (size_t)wv * kTileBytes;
+    double* tx = (double*)tb;            // FP64: x[64], y[64], z[64], w[64], flags[64]
+    double* ty = tx + kWave;
+    double* tz = ty + kWave;
+    float4* tl = (float4*)tb;            // FP32: float4[64], w[64], flags[64]
+    double* tw = FP64 ? (tz + kWave) : (double*)(tl + kWave);
+    uint32_t* tf = (uint32_t*)(tw + kWave);
 
     for (int k = tid; k < nbt * reps; k += kWG) hc[k] = 0;
     if (W)
         for (int k = tid; k < 2 * nbt; k += kWG) hw[k] = 0;
     for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
     for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
-    __shared__ int s_item;
     __syncthreads();
 
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi, ua = B.ua, ub = B.ub;
@@ -212,156 +275,161 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
     const int shift = B.fx_shift2;
     const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
+    const int nrow = ny * nz;
+    const float h32 = a.g.h32;
+    const double h = a.g.h;
 
     Stats st{0, 0, 0};
-    unsigned long long bound = 0;   // upper bound of u32 increments since the last flush
+    unsigned long long bound = 0;   // LDS increments issued by this wave since its last drain
     const int xcd = blockIdx.x & 7;
     int tried = 0;
 
     while (true) {
-        if (tid == 0) {
-            int it = -1;
+        int item = -1;
+        if (lane == 0) {
             while (tried < 8) {
                 const int sidx = (xcd + tried) & 7;
                 const int lo = (int)(((long long)a.n_chunks * sidx) >> 3);
                 const int hi = (int)(((long long)a.n_chunks * (sidx + 1)) >> 3);
                 const int k = atomicAdd(&a.queue[sidx], 1);
-                if (lo + k < hi) { it = lo + k; break; }
+                if (lo + k < hi) { item = lo + k; break; }
                 ++tried;
             }
-            s_item = it;
         }
-        __syncthreads();
-        const int item = s_item;
-        __syncthreads();
+        item = __builtin_amdgcn_readfirstlane(item);
         if (item < 0) break;
 
-        const int A = find_cell(a.chunk_off, a.g.ncell, item);
-        const int q = item - a.chunk_off[A];
-        const int cs = a.cell_start[A], ce = a.cell_start[A + 1];
-        const int i0 = cs + q * kWG;
-        const int i = i0 + tid;
-        const bool vi = i < ce;
-        const int nvalid = min(kWG, ce - i0);
-        const int ax = A % nx, ay = (A / nx) % ny, az = A / (nx * ny);
-        const double ox = a.g.lo[0] + ax * a.g.h, oy = a.g.lo[1] + ay * a.g.h, oz = a.g.lo[2] + az * a.g.h;
+        const int row = find_row(a.row_chunk, nrow, item);
+        const int q = item - a.row_chunk[row];
+        const int cy = row % ny, cz = row / ny;
+        const int rs = a.cell_start[row * nx], re = a.cell_start[(row + 1) * nx];
+        const int i0 = rs + q * kChunk;
+        const int n_i = min(kChunk, re - i0);
+        const int i = i0 + lane;
+        const bool vi = lane < n_i;
+        const int cx_first = (int)a.pl[i0].w;
+        const int cx_last = (int)a.pl[i0 + n_i - 1].w;
 
         double4 Pi = make_double4(0, 0, 0, 0);
+        float4 Li = make_float4(0, 0, 0, 0);
         uint32_t fi = 2u;
-        if (vi) { Pi = a.pw[i]; fi = a.flags[i]; }
-        const float xi = (float)(Pi.x - ox), yi = (float)(Pi.y - oy), zi = (float)(Pi.z - oz);
-        const float ox32 = (float)ox, oy32 = (float)oy, oz32 = (float)oz;
+        if (vi) {
+            Pi = a.pw[i];
+            Li = a.pl[i];
+            fi = a.flags[i];
+        }
         const int ri = fi & 1, ci = (fi & 2) ? 0 : 1;
+        const float ax32 = (float)Pi.x, ay32 = (float)Pi.y, az32 = (float)Pi.z;   // absolute, for mu
 
         for (int r = 0; r < a.nrows; ++r) {
-            const Row row = a.rows[r];
-            const int by = ay + row.dy, bz = az + row.dz;
+            const Row rw = a.rows[r];
+            const int by = cy + rw.dy, bz = cz + rw.dz;
             if (by < 0 || by >= ny || bz < 0 || bz >= nz) continue;
-            const int bx0 = max(ax + row.x0, 0), bx1 = min(ax + row.x1, nx - 1);
+            const int bx0 = max(cx_first + rw.x0, 0), bx1 = min(cx_last + rw.x1, nx - 1);
             if (bx0 > bx1) continue;
-            const bool self = (row.dy == 0 && row.dz == 0);
+            const bool self = (rw.dy == 0 && rw.dz == 0);
             const int rb = (bz * ny + by) * nx;
             int js = a.cell_start[rb + bx0];
             const int je = a.cell_start[rb + bx1 + 1];
             if (self) js = max(js, i0 + 1);
-            for (int jt = js; jt < je; jt += kWG) {
-                const int nt = min(kWG, je - jt);
-                __syncthreads();
-                if (tid < nt) {
-                    const double4 Pj = a.pw[jt + tid];
-                    const uint32_t fj = a.flags[jt + tid];
+            // lane position relative to the neighbour row's cell origin in y and z
+            const float yi_r = Li.y - (float)rw.dy * h32;
+            const float zi_r = Li.z - (float)rw.dz * h32;
+            for (int jt = js; jt < je; jt += kWave) {
+                const int nt = min(kWave, je - jt);
+                wave_sync();
+                if (lane < nt) {
+                    const int j = jt + lane;
                     if constexpr (FP64) {
-                        tx[tid] = Pj.x; ty[tid] = Pj.y; tz[tid] = Pj.z;
-                        tf[tid] = fj;
+                        const double4 Pj = a.pw[j];
+                        tx[lane] = Pj.x; ty[lane] = Pj.y; tz[lane] = Pj.z;
+                        if (W) tw[lane] = ldexp(Pj.w, shift);
                     } else {
-                        tile32[tid] = make_float4((float)(Pj.x - ox), (float)(Pj.y - oy), (float)(Pj.z - oz),
-                                                  __uint_as_float(fj));
+                        tl[lane] = a.pl[j];
+                        if (W) tw[lane] = ldexp(a.pw[j].w, shift);
                     }
-                    if (W) tw[tid] = Pj.w;
+                    tf[lane] = a.flags[j];
                 }
-                __syncthreads();
-                bound += (unsigned long long)nvalid * nt * 2ull;
+                wave_sync();
+                bound += (unsigned long long)nt * 2ull * kWave;
                 int kbeg = 0;
                 if (self) kbeg = max(0, i + 1 - jt);
                 if (!vi) kbeg = nt;
                 st.cand += (unsigned long long)max(0, nt - kbeg);
                 for (int k = kbeg; k < nt; ++k) {
                     int b, m = 0, mr = 0;
-                    uint32_t fj;
+                    const uint32_t fj = tf[k];
                     if constexpr (FP64) {
-                        const double4 Pj = make_double4(tx[k], ty[k], tz[k], 0.0);
-                        const double s = exact_s(Pi, Pj);
+                        const double jx = tx[k], jy = ty[k], jz = tz[k];
+                        const double s = exact_s(Pi.x, Pi.y, Pi.z, jx, jy, jz);
                         if (!(s > t_lo && s < t_hi)) continue;
                         const float sf = (float)s;
-                        float u = logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub)
-                                       : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub);
+                        const float u = logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub)
+                                             : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub);
                         b = (int)u;
                         b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
                         while (b > 0 && s < t64[b]) --b;
                         while (b < nb && s >= t64[b + 1]) ++b;
-                        if (b == nb) { st.over += (uint64_t)(ci + (((tf[k] & 2) == 0) ? 1 : 0)); continue; }
-                        if constexpr (RSD) exact_mu(Pi, Pj, ms, nm, m, mr);
-                        fj = tf[k];
+                        if (b == nb) { st.over += (uint64_t)(ci + ((fj & 2) ? 0 : 1)); continue; }
+                        if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, jx, jy, jz, ms, nm, m, mr);
                     } else {
-                        const float4 tj = tile32[k];
-                        const float dx = tj.x - xi, dy = tj.y - yi, dz = tj.z - zi;
+                        const float4 tj = tl[k];
+                        const float dx = fmaf(tj.w - Li.w, h32, tj.x - Li.x);
+                        const float dy = tj.y - yi_r, dz = tj.z - zi_r;
                         const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         if (!(s > rej_lo && s < rej_hi)) continue;
-                        float u = logb ? fmaf(__builtin_amdgcn_logf(s), ua, ub)
-                                       : fmaf(__builtin_amdgcn_sqrtf(s), ua, ub);
+                        const float u = logb ? fmaf(__builtin_amdgcn_logf(s), ua, ub)
+                                             : fmaf(__builtin_amdgcn_sqrtf(s), ua, ub);
                         b = (int)u;
                         b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
                         const float2 ok = okb[b];
                         bool safe = (s >= ok.x) && (s < ok.y);
                         if constexpr (RSD) {
-                            const float mx = fmaf(0.5f, xi + tj.x, ox32);
-                            const float my = fmaf(0.5f, yi + tj.y, oy32);
-                            const float mz = fmaf(0.5f, zi + tj.z, oz32);
-                            // sep = i - j = -(dx, dy, dz)
+                            // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
+                            const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
                             const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
                             const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
-                            const float rm = __builtin_amdgcn_rsqf(M2), rs = __builtin_amdgcn_rsqf(s);
-                            const float th = D * rm * rs;
+                            const float rm = __builtin_amdgcn_rsqf(M2), rsq = __builtin_amdgcn_rsqf(s);
+                            const float th = D * rm * rsq;
                             const float t = fmaf(th, ms32, ms32);
                             const float tfl = floorf(t);
                             const float fr = t - tfl;
-                            const float dt = ms32 * fmaf(mu_ew, rs, fmaf(mu_em, rm, mu_c));
+                            const float dt = ms32 * fmaf(mu_ew, rsq, fmaf(mu_em, rm, mu_c));
                             const int mi = (int)tfl;
                             const bool near_lo = (fr < dt) && (mi >= 1) && (mi <= nm - 1);
-                            const bool near_hi = (fr > 1.0f - dt) && (mi + 1 >= 1) && (mi + 1 <= nm - 1);
+                            const bool near_hi = (fr > 1.0f - dt) && (mi >= 0) && (mi <= nm - 2);
                             // dt is not finite when mid or sep is ~0: always re-evaluate
                             safe = safe && !near_lo && !near_hi && (dt < 0.25f);
                             m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
                             mr = nm - 1 - m;
                         }
-                        fj = __float_as_uint(tj.w);
                         if (!safe) {
                             ++st.rechk;
                             const double4 Pj = a.pw[jt + k];
-                            const double se = exact_s(Pi, Pj);
+                            const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
                             if (!(se > t_lo && se < t_hi)) continue;
                             b = exact_bin(se, t64, nb);
-                            if (b == nb) { st.over += (uint64_t)(ci + (((fj & 2) == 0) ? 1 : 0)); continue; }
-                            if constexpr (RSD) exact_mu(Pi, Pj, ms, nm, m, mr);
+                            if (b == nb) { st.over += (uint64_t)(ci + ((fj & 2) ? 0 : 1)); continue; }
+                            if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
                         }
                     }
                     const int rj = fj & 1, cj = (fj & 2) ? 0 : 1;
                     double pwt = 0.0;
                     if constexpr (W) pwt = Pi.w * tw[k];
-                    accumulate<RSD, W>(hc, hw, reps, rep, nb, nm, ri + rj, b, m, mr, ci, cj, pwt, shift);
+                    accumulate<RSD, W>(hc, hw, reps, rep, nb, nm, mirror, ri + rj, b, m, mr, ci, cj, pwt, shift,
+                                       a.out_cnt, a.out_w);
                 }
-                // keep every u32 LDS counter below 2^32 (bound is uniform over the work-group)
-                if (bound > (1ull << 31)) {
-                    __syncthreads();
-                    flush(hc, hw, nbt, reps, W, a.out_cnt, a.out_w);
-                    __syncthreads();
+                // keep every u32 LDS counter (and the split sums) below 2^32: each wave drains
+                // after 2^29 of its own increments, so four waves stay below 2^31 in total
+                if (bound > (1ull << 29)) {
+                    drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt, a.out_w);
                     bound = 0;
                 }
             }
         }
     }
     __syncthreads();
-    flush(hc, hw, nbt, reps, W, a.out_cnt, a.out_w);
+    drain(hc, hw, nbt, nm, reps, W, mirror, tid, kWG, a.out_cnt, a.out_w);
 
     // wave-reduce the statistics
     for (int off = 32; off > 0; off >>= 1) {
@@ -369,7 +437,7 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
         st.rechk += __shfl_down(st.rechk, off, 64);
         st.over += __shfl_down(st.over, off, 64);
     }
-    if ((tid & 63) == 0) {
+    if (lane == 0) {
         if (st.cand) atomicAdd(&a.out_stats[0], st.cand);
         if (st.rechk) atomicAdd(&a.out_stats[1], st.rechk);
         if (st.over) atomicAdd(&a.out_stats[2], st.over);

@@ -1,0 +1,335 @@
+// pcf34.cpp -- C ABI for the 3PCF and 4PCF paths (gs_count_3pcf, gs_count_4pcf):
+// grid, stencil, configuration tables, kernel launch and the host-side
+// combination of exact class counts / sums into the reference's N2, N3,
+// discN, discR.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gs_api_internal.h"
+#include "gs_kernels.h"
+
+using namespace gs;
+
+namespace gs {
+
+void extra_release(GsExtra& x)
+{
+    x.params.release();
+    x.out.release();
+    x.queue.release();
+    x.dbl.release();
+    x.i32.release();
+}
+
+// bintable for the 3PCF (gramsci.F90:1459-1474), 0-based configuration of a
+// sorted triple in the i <= j <= k enumeration, for every permutation.
+static std::vector<uint16_t> table3(int nb)
+{
+    std::vector<uint16_t> t((size_t)nb * nb * nb, 0);
+    int cfg = 0;
+    for (int i = 0; i < nb; ++i)
+        for (int j = i; j < nb; ++j)
+            for (int k = j; k < nb; ++k) {
+                const int p[6][3] = {{i, j, k}, {i, k, j}, {j, i, k}, {j, k, i}, {k, i, j}, {k, j, i}};
+                for (auto& q : p) t[((size_t)q[0] * nb + q[1]) * nb + q[2]] = (uint16_t)cfg;
+                ++cfg;
+            }
+    return t;
+}
+
+// bintable for the 4PCF (gramsci.F90:1402-1420): 1-based configuration for
+// non-decreasing quadruples only, 0 elsewhere.
+static std::vector<uint16_t> table4(int nb)
+{
+    std::vector<uint16_t> t((size_t)nb * nb * nb * nb, 0);
+    int cfg = 0;
+    for (int i = 0; i < nb; ++i)
+        for (int j = i; j < nb; ++j)
+            for (int k = j; k < nb; ++k)
+                for (int l = k; l < nb; ++l) t[(((size_t)i * nb + j) * nb + k) * nb + l] = (uint16_t)(++cfg);
+    return t;
+}
+
+static void fill_stats(gs_stats* st, hipEvent_t* ev, const unsigned long long* s, int m)
+{
+    if (!st) return;
+    memset(st, 0, sizeof(*st));
+    float ms_grid = 0, ms_count = 0;
+    (void)hipEventElapsedTime(&ms_grid, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&ms_count, ev[2], ev[3]);
+    st->ms_grid = ms_grid;
+    st->ms_count = ms_count;
+    st->ms_kernel_avg = ms_count;
+    st->n_candidates = s[0];
+    st->n_recheck = s[1];
+    st->n_overflow = s[2];
+    st->n_units = s[3];
+    st->n_binned = s[4];
+    st->cell_div = m;
+}
+
+}  // namespace gs
+
+// ====================================================================== 3PCF
+
+extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2, double* n3,
+                             gs_stats* st)
+{
+    if (!c) return fail(GS_EINVAL, "NULL context");
+    int rc = check_params(p, 3);
+    if (rc) return rc;
+    if ((rc = ctx_set_device(c))) return rc;
+    const int nb = p->nbins;
+    const int ncfg = nb * (nb + 1) * (nb + 2) / 6;
+    const int nh = ncfg * 4;
+    if (nh > 16384) return fail(GS_ELIMIT, "3PCF with %d bins has too many configurations for the LDS histogram", nb);
+    const bool fp64 = p->precision == GS_FP64;
+    CtxView v = view(c);
+    const bool weighted = v.has_w;
+    const int m = p->cell_div > 0 ? p->cell_div : 2;
+
+    (void)hipEventRecord(v.ev[0], v.stream);
+    double h;
+    if ((rc = ctx_build_grid(c, p->rmax, m, &h))) return rc;
+    (void)hipEventRecord(v.ev[1], v.stream);
+    v = view(c);
+    const int mst = (int)std::ceil(p->rmax / h - 1e-12);
+    std::vector<Row> rows = full_stencil(h, p->rmax, mst);
+    BinDesc B;
+    if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
+    std::vector<uint16_t> tab = table3(nb);
+
+    int reps = 1;
+    while (reps < 32 && (size_t)nh * reps * 2 * 4 <= 16 * 1024) reps *= 2;
+    int kmax = 4096;
+    const size_t budget = 64 * 1024;
+    while (kmax > 256 && triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted) > budget) kmax -= 256;
+    while (reps > 1 && triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted) > budget) reps /= 2;
+    if (triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted) > budget)
+        return fail(GS_ELIMIT, "3PCF tables do not fit in LDS");
+
+    GsExtra& X = *v.extra;
+    const size_t tab_off = ((sizeof(BinDesc) + 15) / 16) * 16;
+    const size_t rows_off = tab_off + ((tab.size() * 2 + 15) / 16) * 16;
+    const size_t nout = (size_t)nh + 2 * (size_t)nh + 8;
+    if ((rc = X.params.reserve(rows_off + rows.size() * sizeof(Row) + 16)) || (rc = X.out.reserve(nout)) ||
+        (rc = X.queue.reserve(16)))
+        return rc;
+    hipError_t e = hipMemcpyAsync(X.params.ptr, &B, sizeof(B), hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(X.params.ptr + tab_off, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess && !rows.empty())
+        e = hipMemcpyAsync(X.params.ptr + rows_off, rows.data(), rows.size() * sizeof(Row), hipMemcpyHostToDevice,
+                           v.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(X.out.ptr, 0, nout * sizeof(unsigned long long), v.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(X.queue.ptr, 0, 16 * sizeof(int32_t), v.stream);
+    if (e != hipSuccess) return hip_fail(e, "3PCF parameter upload");
+
+    TripArgs a{};
+    a.pw = v.pw;
+    a.flags = v.flags;
+    a.cell_start = v.cell_start;
+    a.rows = (const Row*)(X.params.ptr + rows_off);
+    a.bins = (const BinDesc*)X.params.ptr;
+    a.cfg_tab = (const uint16_t*)(X.params.ptr + tab_off);
+    a.queue = X.queue.ptr;
+    a.err = X.queue.ptr + 8;
+    a.out_cnt = X.out.ptr;
+    a.out_w = X.out.ptr + nh;
+    a.out_stats = X.out.ptr + 3 * nh;
+    a.g = v.g;
+    a.g.m = mst;
+    a.nrows = (int)rows.size();
+    a.n_items = (int)(v.n_data + v.n_rand);
+    a.ncfg = ncfg;
+    a.reps = reps;
+    a.kmax = kmax;
+    const int occ = triplet3_occupancy(nb, ncfg, reps, kmax, fp64, weighted);
+    int nblocks = std::max(1, std::min(v.n_cu * occ, a.n_items));
+    (void)hipEventRecord(v.ev[2], v.stream);
+    if (a.n_items > 0) {
+        e = launch_triplet3(a, nb, fp64, weighted, nblocks, v.stream);
+        if (e != hipSuccess) return hip_fail(e, "3PCF kernel launch");
+    }
+    (void)hipEventRecord(v.ev[3], v.stream);
+    std::vector<unsigned long long> h_out(nout);
+    e = hipMemcpyAsync(h_out.data(), X.out.ptr, nout * sizeof(unsigned long long), hipMemcpyDeviceToHost, v.stream);
+    int32_t err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&err, X.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+    if (e != hipSuccess) return hip_fail(e, "3PCF kernel");
+    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", kmax);
+
+    std::vector<long double> S(nh);
+    for (int k = 0; k < nh; ++k) {
+        if (cnt) cnt[k] = h_out[k];
+        S[k] = weighted ? fx_value(h_out[nh + 2 * k], h_out[nh + 2 * k + 1], B.fx_shift3) : (long double)h_out[k];
+        if (wsum) wsum[k] = (double)S[k];
+    }
+    // N2 = sum over classes of (-1)^r S_r / (Wd^(3-r) Wr^r); N3 = S_3 / Wr^3 (gramsci.F90:505-507)
+    const long double Wd = v.wsum_d, Wr = v.wsum_r;
+    for (int q = 0; q < ncfg; ++q) {
+        long double v2 = 0, v3 = 0;
+        for (int r = 0; r <= 3; ++r) {
+            const long double s = S[q * 4 + r];
+            if (s == 0) continue;
+            long double den = 1;
+            for (int t = 0; t < 3 - r; ++t) den *= Wd;
+            for (int t = 0; t < r; ++t) den *= Wr;
+            v2 += ((r & 1) ? -s : s) / den;
+        }
+        if (Wr > 0) v3 = S[q * 4 + 3] / (Wr * Wr * Wr);
+        if (n2) n2[q] = (double)v2;
+        if (n3) n3[q] = (double)v3;
+    }
+    fill_stats(st, v.ev, h_out.data() + 3 * nh, mst);
+    return GS_OK;
+}
+
+// ====================================================================== 4PCF
+
+extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* n2, double* n3, double* disc_n,
+                             double* disc_r, gs_stats* st)
+{
+    if (!c) return fail(GS_EINVAL, "NULL context");
+    int rc = check_params(p, 4);
+    if (rc) return rc;
+    if ((rc = ctx_set_device(c))) return rc;
+    const int nb = p->nbins;
+    if (nb > 16) return fail(GS_ELIMIT, "4PCF supports nbins <= 16 (got %d)", nb);
+    const int ncfg = nb * (nb + 1) * (nb + 2) * (nb + 3) / 24;
+    const int ncell = ncfg * nb * nb;
+    CtxView v = view(c);
+    const bool weighted = v.has_w;
+    const int m = p->cell_div > 0 ? p->cell_div : 2;
+
+    (void)hipEventRecord(v.ev[0], v.stream);
+    double h;
+    if ((rc = ctx_build_grid(c, p->rmax, m, &h))) return rc;
+    (void)hipEventRecord(v.ev[1], v.stream);
+    v = view(c);
+    const int mst = (int)std::ceil(p->rmax / h - 1e-12);
+    std::vector<Row> rows = full_stencil(h, p->rmax, mst);
+    BinDesc B;
+    if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
+    std::vector<uint16_t> tab = table4(nb);
+
+    const size_t budget = 64 * 1024;
+    int kmax = 256;
+    bool lds_hist = (size_t)ncell * 64 <= 16 * 1024;
+    while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) kmax /= 2;
+    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) {
+        lds_hist = false;
+        kmax = 256;
+        while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) kmax /= 2;
+    }
+    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) return fail(GS_ELIMIT, "4PCF tables do not fit in LDS");
+
+    GsExtra& X = *v.extra;
+    const size_t tab_off = ((sizeof(BinDesc) + 15) / 16) * 16;
+    const size_t rows_off = tab_off + ((tab.size() * 2 + 15) / 16) * 16;
+    const size_t ncnt = (size_t)ncell * 16;
+    const size_t nout = ncnt + 8;
+    const size_t nw = weighted ? (size_t)ncell * 16 + (size_t)6 * ncell * 4 : 0;
+    if ((rc = X.params.reserve(rows_off + rows.size() * sizeof(Row) + 16)) || (rc = X.out.reserve(nout)) ||
+        (rc = X.queue.reserve(16)) || (rc = X.dbl.reserve(std::max<size_t>(nw, 1))))
+        return rc;
+    hipError_t e = hipMemcpyAsync(X.params.ptr, &B, sizeof(B), hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(X.params.ptr + tab_off, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess && !rows.empty())
+        e = hipMemcpyAsync(X.params.ptr + rows_off, rows.data(), rows.size() * sizeof(Row), hipMemcpyHostToDevice,
+                           v.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(X.out.ptr, 0, nout * sizeof(unsigned long long), v.stream);
+    if (e == hipSuccess && nw) e = hipMemsetAsync(X.dbl.ptr, 0, nw * sizeof(double), v.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(X.queue.ptr, 0, 16 * sizeof(int32_t), v.stream);
+    if (e != hipSuccess) return hip_fail(e, "4PCF parameter upload");
+
+    QuadArgs a{};
+    a.pw = v.pw;
+    a.flags = v.flags;
+    a.orig = v.orig;
+    a.cell_start = v.cell_start;
+    a.rows = (const Row*)(X.params.ptr + rows_off);
+    a.bins = (const BinDesc*)X.params.ptr;
+    a.cfg_tab = (const uint16_t*)(X.params.ptr + tab_off);
+    a.queue = X.queue.ptr;
+    a.err = X.queue.ptr + 8;
+    a.out_cnt = X.out.ptr;
+    a.out_w = X.dbl.ptr;
+    a.out_stats = X.out.ptr + ncnt;
+    a.g = v.g;
+    a.g.m = mst;
+    a.nrows = (int)rows.size();
+    a.n_items = (int)(v.n_data + v.n_rand);
+    a.ncell = ncell;
+    a.kmax = kmax;
+    a.lds_hist = lds_hist ? 1 : 0;
+    const int occ = quad4_occupancy(nb, ncell, kmax, lds_hist, weighted);
+    int nblocks = std::max(1, std::min(v.n_cu * occ, a.n_items));
+    (void)hipEventRecord(v.ev[2], v.stream);
+    if (a.n_items > 0) {
+        e = launch_quad4(a, nb, weighted, nblocks, v.stream);
+        if (e != hipSuccess) return hip_fail(e, "4PCF kernel launch");
+    }
+    (void)hipEventRecord(v.ev[3], v.stream);
+    std::vector<unsigned long long> h_out(nout);
+    std::vector<double> h_w(nw);
+    e = hipMemcpyAsync(h_out.data(), X.out.ptr, nout * sizeof(unsigned long long), hipMemcpyDeviceToHost, v.stream);
+    if (e == hipSuccess && nw)
+        e = hipMemcpyAsync(h_w.data(), X.dbl.ptr, nw * sizeof(double), hipMemcpyDeviceToHost, v.stream);
+    int32_t err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&err, X.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+    if (e != hipSuccess) return hip_fail(e, "4PCF kernel");
+    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", kmax);
+
+    if (cnt) memcpy(cnt, h_out.data(), ncnt * sizeof(uint64_t));
+    // normalised weight of one point: +w/Wd (data) or -w/Wr (random) (gramsci.F90:1390-1391)
+    const long double Wd = v.wsum_d, Wr = v.wsum_r;
+    auto norm = [&](int r) -> long double { return r ? -1.0L / Wr : 1.0L / Wd; };
+    for (int q = 0; q < ncell; ++q) {
+        long double v2 = 0, v3 = 0;
+        for (int pat = 0; pat < 16; ++pat) {
+            const long double s = weighted ? (long double)h_w[(size_t)q * 16 + pat] : (long double)h_out[(size_t)q * 16 + pat];
+            if (s == 0) continue;
+            const long double f = norm(pat & 1) * norm((pat >> 1) & 1) * norm((pat >> 2) & 1) * norm((pat >> 3) & 1);
+            v2 += s * f;
+            if (pat == 15) v3 += s * f;
+        }
+        if (n2) n2[q] = (double)v2;
+        if (n3) n3[q] = (double)v3;
+    }
+    // disc terms: t -> (points a, b of the pair product; point c that must be random)
+    //   t0: (i,k1 | k3)  t1: (i,k2 | k3)  t2: (i,k3 | k1)  t3: (k1,k2 | i)  t4: (k3,k2 | i)  t5: (k3,k1 | i)
+    static const int ta[6] = {0, 0, 0, 1, 3, 3}, tb[6] = {1, 2, 3, 2, 2, 1}, tc[6] = {3, 3, 1, 0, 0, 0};
+    for (int t = 0; t < 6; ++t)
+        for (int q = 0; q < ncell; ++q) {
+            long double dn = 0, dr = 0;
+            for (int ab = 0; ab < 4; ++ab) {
+                const int ra = ab & 1, rb = (ab >> 1) & 1;
+                long double s;
+                if (weighted) {
+                    s = h_w[(size_t)ncell * 16 + ((size_t)t * ncell + q) * 4 + ab];
+                } else {
+                    s = 0;
+                    for (int pat = 0; pat < 16; ++pat)
+                        if (((pat >> tc[t]) & 1) && ((pat >> ta[t]) & 1) == ra && ((pat >> tb[t]) & 1) == rb)
+                            s += (long double)h_out[(size_t)q * 16 + pat];
+                }
+                if (s == 0) continue;
+                const long double term = s * norm(ra) * norm(rb);
+                dn += term;
+                if (ra && rb) dr += term;
+            }
+            if (disc_n) disc_n[(size_t)t * ncell + q] = (double)dn;
+            if (disc_r) disc_r[(size_t)t * ncell + q] = (double)dr;
+        }
+    std::vector<unsigned long long> sts(5, 0);
+    for (int k = 0; k < 5; ++k) sts[k] = h_out[ncnt + k];
+    fill_stats(st, v.ev, sts.data(), mst);
+    return GS_OK;
+}

@@ -1,0 +1,306 @@
+// quad4.hip -- persistent 4PCF quad kernel.
+//
+// Replaces create_graph (gramsci.F90:288-348) + query_4pcf_all2 (:662-861).
+// For every apex i that is not a buffer point and has more than 3 kept
+// neighbours (:675), and every k1 < k2 < k3 of its neighbours in ascending
+// ORIGINAL index (the reference's neighbour lists are sorted by id,
+// sorting_module.F90:36-47), with all three cross sides kept
+// (find_dist :685,692,694), the quad is binned at
+//     cell = (bintable(ind1, ind4, ind5, ind3), ind2, ind6)
+// where ind1..3 are the apex sides, ind4 = (k1,k2), ind5 = (k2,k3) and
+// ind6 = (k1,k3); bintable is only filled for non-decreasing perimeters
+// (:1402-1420), so other orientations are dropped (:701).
+//
+// One work-group per apex, persistent over apexes:
+//   1. gather the kept neighbours (exact FP64 bins) into LDS;
+//   2. sort them by original index (bitonic sort in LDS);
+//   3. tabulate every neighbour-neighbour side bin in a K x K LDS matrix;
+//   4. enumerate linked (k1, k2) pairs over the lanes, each lane scanning k3.
+// Counts go to an LDS histogram [cell][16 random patterns] (or straight to
+// global memory when it does not fit); weighted runs add FP64 sums of the
+// quad products per pattern and of the six disc pair products (:754-818).
+#include "gs_kernels.h"
+
+namespace gs {
+
+namespace {
+
+__device__ __forceinline__ double exact_s(double ax, double ay, double az, double bx, double by, double bz)
+{
+    double dx = __dsub_rn(bx, ax), dy = __dsub_rn(by, ay), dz = __dsub_rn(bz, az);
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// exact 0-based bin, -1 if not kept, nb if the reference bin would overflow
+__device__ __forceinline__ int bin_of(double s, const double* t, int nb)
+{
+    if (!(s > t[0] && s < t[nb + 1])) return -1;
+    int lo = 0, hi = nb;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (t[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist)
+{
+    size_t b = 0;
+    if (lds_hist) b += (size_t)ncell * 16 * 4;
+    b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;            // t64
+    b += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;          // perimeter table (u16)
+    b += (size_t)kmax * (24 + 8 + 4 + 4 + 4);                     // x, y, z, w, key, slot/packed, sorted j
+    b += ((size_t)kmax * kmax + 15) / 16 * 16;                    // link matrix (u8: bin+1, 0 = not kept)
+    return b;
+}
+
+template <bool W>
+__global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BinDesc& B = *a.bins;
+    const int nb = B.nb;
+    const int kmax = a.kmax;
+    const int ncell = a.ncell;
+    const bool lh = a.lds_hist != 0;
+    const int tid = threadIdx.x;
+
+    unsigned char* p = smem;
+    uint32_t* hc = (uint32_t*)p;
+    if (lh) p += (size_t)ncell * 16 * 4;
+    double* t64 = (double*)p;
+    p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
+    uint16_t* tab = (uint16_t*)p;
+    p += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;
+    double* nx_ = (double*)p;
+    double* ny_ = nx_ + kmax;
+    double* nz_ = ny_ + kmax;
+    double* nw_ = nz_ + kmax;
+    int32_t* key = (int32_t*)(nw_ + kmax);   // original index (sort key)
+    int32_t* pk = key + kmax;                // sort permutation, then bin | rand << 8 in sorted order
+    int32_t* jl = pk + kmax;                 // bin | rand << 8 in gather order
+    uint8_t* L = (uint8_t*)(jl + kmax);      // L[u * kmax + v]
+
+    __shared__ int s_item, s_k, s_over;
+
+    if (lh)
+        for (int k = tid; k < ncell * 16; k += kWG) hc[k] = 0;
+    for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
+    for (int k = tid; k < nb * nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
+    __syncthreads();
+
+    const int nxg = a.g.dim[0], nyg = a.g.dim[1], nzg = a.g.dim[2];
+    const double inv_h = 1.0 / a.g.h;
+    unsigned long long units = 0, binned = 0, over = 0, cand = 0;
+    const int xcd = blockIdx.x & 7;
+    int tried = 0;
+    const int nb2 = nb * nb;
+
+    while (true) {
+        if (tid == 0) {
+            int it = -1;
+            while (tried < 8) {
+                const int sidx = (xcd + tried) & 7;
+                const int lo = (int)(((long long)a.n_items * sidx) >> 3);
+                const int hi = (int)(((long long)a.n_items * (sidx + 1)) >> 3);
+                const int k = atomicAdd(&a.queue[sidx], 1);
+                if (lo + k < hi) { it = lo + k; break; }
+                ++tried;
+            }
+            s_item = it;
+            s_k = 0;
+            s_over = 0;
+        }
+        __syncthreads();
+        const int apex = s_item;
+        if (apex < 0) break;
+        const uint32_t fa = a.flags[apex];
+        if (fa & 2u) {
+            __syncthreads();
+            continue;
+        }
+        const double4 Pa = a.pw[apex];
+        const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_h), 0), nxg - 1);
+        const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), nyg - 1);
+        const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nzg - 1);
+
+        // ---- 1. gather
+        for (int r = 0; r < a.nrows; ++r) {
+            const Row rw = a.rows[r];
+            const int by = cy + rw.dy, bz = cz + rw.dz;
+            if (by < 0 || by >= nyg || bz < 0 || bz >= nzg) continue;
+            const int bx0 = max(cx + rw.x0, 0), bx1 = min(cx + rw.x1, nxg - 1);
+            if (bx0 > bx1) continue;
+            const int rb = (bz * nyg + by) * nxg;
+            const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
+            for (int j = js + tid; j < je; j += kWG) {
+                ++cand;
+                const double4 Pj = a.pw[j];
+                const int b = bin_of(exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z), t64, nb);
+                if (b < 0) continue;
+                if (b == nb) { ++over; continue; }
+                const int pos = atomicAdd(&s_k, 1);
+                if (pos >= kmax) { s_over = 1; continue; }
+                nx_[pos] = Pj.x; ny_[pos] = Pj.y; nz_[pos] = Pj.z; nw_[pos] = Pj.w;
+                key[pos] = a.orig[j];
+                jl[pos] = (int)(b | ((a.flags[j] & 1u) << 8));   // packed, moved by the sort below
+            }
+        }
+        __syncthreads();
+        const int K = min(s_k, kmax);
+        if (s_over && tid == 0) atomicExch(a.err, 1);
+        if (K <= 3) {        // gramsci.F90:675
+            __syncthreads();
+            continue;
+        }
+
+        // ---- 2. bitonic sort of (key, slot) by original index
+        int np2 = 1;
+        while (np2 < K) np2 <<= 1;
+        for (int k = tid; k < np2; k += kWG) pk[k] = k;           // slot permutation
+        __syncthreads();
+        for (int size = 2; size <= np2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = tid; t < np2 / 2; t += kWG) {
+                    const int lo = 2 * t - (t & (stride - 1));
+                    const int hi = lo + stride;
+                    const bool up = ((lo & size) == 0);
+                    const int sa = pk[lo], sb = pk[hi];
+                    const int ka = sa < K ? key[sa] : 0x7fffffff;
+                    const int kb = sb < K ? key[sb] : 0x7fffffff;
+                    if ((ka > kb) == up) { pk[lo] = sb; pk[hi] = sa; }
+                }
+                __syncthreads();
+            }
+        }
+        // gather into sorted order: slot pk[u] -> position u (stage through the link-matrix area)
+        double* tmp = (double*)L;   // L is not in use yet; needs 5*K*8 bytes <= kmax*kmax
+        for (int u = tid; u < K; u += kWG) {
+            const int sl = pk[u];
+            tmp[u] = nx_[sl];
+            tmp[K + u] = ny_[sl];
+            tmp[2 * K + u] = nz_[sl];
+            tmp[3 * K + u] = nw_[sl];
+            ((int*)(tmp + 4 * K))[u] = jl[sl];
+        }
+        __syncthreads();
+        for (int u = tid; u < K; u += kWG) {
+            nx_[u] = tmp[u]; ny_[u] = tmp[K + u]; nz_[u] = tmp[2 * K + u]; nw_[u] = tmp[3 * K + u];
+            pk[u] = ((int*)(tmp + 4 * K))[u];
+        }
+        __syncthreads();
+
+        // ---- 3. link matrix of neighbour-neighbour side bins (find_dist)
+        for (int e = tid; e < K * K; e += kWG) {
+            const int u = e / K, v = e - u * K;
+            uint8_t val = 0;
+            if (u < v) {
+                const int b = bin_of(exact_s(nx_[u], ny_[u], nz_[u], nx_[v], ny_[v], nz_[v]), t64, nb);
+                val = (b >= 0 && b < nb) ? (uint8_t)(b + 1) : 0;
+            }
+            L[u * K + v] = val;
+        }
+        __syncthreads();
+
+        // ---- 4. quads: linked (k1, k2) over the lanes, k3 > k2 scanned by the lane
+        const int ri = (int)(fa & 1u);
+        const double wi = Pa.w;
+        const long long P = (long long)K * (K - 1) / 2;
+        int u = 0;
+        long long rem = tid;
+        while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
+        for (long long pi = tid; pi < P; pi += kWG) {
+            const int v = u + 1 + (int)rem;
+            const int ind4 = L[u * K + v];
+            if (ind4) {
+                units += (unsigned long long)(K - 1 - v);
+                const int p1 = pk[u], p2 = pk[v];
+                const int ind1 = (p1 & 255) + 1, ind2 = (p2 & 255) + 1;
+                const int r1 = (p1 >> 8) & 1, r2 = (p2 >> 8) & 1;
+                for (int k3 = v + 1; k3 < K; ++k3) {
+                    const int ind5 = L[v * K + k3];
+                    if (!ind5) continue;
+                    const int ind6 = L[u * K + k3];
+                    if (!ind6) continue;
+                    const int p3 = pk[k3];
+                    const int ind3 = (p3 & 255) + 1;
+                    const int cfg = tab[(((ind1 - 1) * nb + (ind4 - 1)) * nb + (ind5 - 1)) * nb + (ind3 - 1)];
+                    if (cfg == 0) continue;
+                    const int r3 = (p3 >> 8) & 1;
+                    const int cell = (cfg - 1) * nb2 + (ind2 - 1) * nb + (ind6 - 1);
+                    const int pat = ri | (r1 << 1) | (r2 << 2) | (r3 << 3);
+                    ++binned;
+                    if (lh) atomicAdd(&hc[cell * 16 + pat], 1u);
+                    else atomicAdd(&a.out_cnt[(size_t)cell * 16 + pat], 1ull);
+                    if constexpr (W) {
+                        const double w1 = nw_[u], w2 = nw_[v], w3 = nw_[k3];
+                        double* ow = a.out_w;
+                        atomicAdd(&ow[(size_t)cell * 16 + pat], wi * w1 * w2 * w3);   // gramsci.F90:710
+                        double* od = ow + (size_t)ncell * 16;                       // [6][ncell][4]
+                        // disc pair products and their conditions, gramsci.F90:754-818
+                        if (ri) {
+                            atomicAdd(&od[((size_t)3 * ncell + cell) * 4 + (r1 | r2 << 1)], w1 * w2);
+                            atomicAdd(&od[((size_t)4 * ncell + cell) * 4 + (r3 | r2 << 1)], w3 * w2);
+                            atomicAdd(&od[((size_t)5 * ncell + cell) * 4 + (r3 | r1 << 1)], w3 * w1);
+                        }
+                        if (r3) {
+                            atomicAdd(&od[((size_t)1 * ncell + cell) * 4 + (ri | r2 << 1)], wi * w2);
+                            atomicAdd(&od[((size_t)0 * ncell + cell) * 4 + (ri | r1 << 1)], wi * w1);
+                        }
+                        if (r1) atomicAdd(&od[((size_t)2 * ncell + cell) * 4 + (ri | r3 << 1)], wi * w3);
+                    }
+                }
+            }
+            rem += kWG;
+            while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (lh)
+        for (int k = tid; k < ncell * 16; k += kWG)
+            if (hc[k]) atomicAdd(&a.out_cnt[k], (unsigned long long)hc[k]);
+    for (int off = 32; off > 0; off >>= 1) {
+        units += __shfl_down(units, off, 64);
+        binned += __shfl_down(binned, off, 64);
+        over += __shfl_down(over, off, 64);
+        cand += __shfl_down(cand, off, 64);
+    }
+    if ((tid & 63) == 0) {
+        if (cand) atomicAdd(&a.out_stats[0], cand);
+        if (over) atomicAdd(&a.out_stats[2], over);
+        if (units) atomicAdd(&a.out_stats[3], units);
+        if (binned) atomicAdd(&a.out_stats[4], binned);
+    }
+}
+
+template <bool W>
+static hipError_t launch_q(const QuadArgs& a, size_t lds, int nblocks, hipStream_t st)
+{
+    auto kern = k_quad4<W>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kWG), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st)
+{
+    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0);
+    return weighted ? launch_q<true>(a, lds, nblocks, st) : launch_q<false>(a, lds, nblocks, st);
+}
+
+int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool weighted)
+{
+    const size_t lds = quad4_lds_bytes(nb, ncell, kmax, lds_hist);
+    int n = 0;
+    hipError_t e = weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_quad4<true>, kWG, lds)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_quad4<false>, kWG, lds);
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+}  // namespace gs

@@ -1,0 +1,347 @@
+// triplet3.hip -- persistent 3PCF triplet kernel (all configurations).
+//
+// Replaces create_graph (gramsci.F90:288-348) + query_graph_3pcf_all
+// (:463-545) with find_dist (:1579-1609).  For every apex i that is not a
+// buffer point, and every unordered pair {k1, k2} of its kept neighbours
+// whose third side (k1, k2) is also kept, the triangle is counted in the
+// configuration of its sorted side bins (bintable, :1459-1474) and in the
+// class given by the number of randoms among (i, k1, k2).  Each triangle is
+// thus counted once per apex (3 times in total), as in the reference.
+//
+// One work-group per apex, persistent over apexes in cell order (8 XCD
+// queues as in the pair kernel):
+//   1. gather: the work-group scans the apex's full stencil of cell rows and
+//      appends every neighbour with rmin < d < rmax to an LDS list, with its
+//      exact FP64 bin (side 1 / side 2 of the reference);
+//   2. pairs: the K(K-1)/2 neighbour pairs are spread evenly over the 256
+//      lanes (flat pair index, advanced by 256 per step); the third side is
+//      evaluated directly from the neighbours' coordinates - in FP64, or in
+//      FP32 relative to the apex with the same guard band and FP64 recheck as
+//      the pair kernel - instead of a binary search in a neighbour list.
+#include "gs_kernels.h"
+
+namespace gs {
+
+namespace {
+
+__device__ __forceinline__ void glb_add128(unsigned long long* slot, unsigned long long lo, unsigned long long hi)
+{
+    unsigned long long old = atomicAdd(slot, lo);
+    if (old + lo < old) hi += 1ull;
+    if (hi) atomicAdd(slot + 1, hi);
+}
+
+__device__ __forceinline__ double exact_s(double ax, double ay, double az, double bx, double by, double bz)
+{
+    double dx = __dsub_rn(bx, ax), dy = __dsub_rn(by, ay), dz = __dsub_rn(bz, az);
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+__device__ __forceinline__ int exact_bin(double s, const double* t, int nb)
+{
+    int lo = 0, hi = nb;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (t[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void quantise(double x, uint32_t& lo, uint32_t& hi)
+{
+    const double h = floor(x * 0x1p-32);
+    hi = (uint32_t)h;
+    lo = (uint32_t)fma(-h, 0x1p32, x);
+}
+
+}  // namespace
+
+size_t triplet3_lds_bytes(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted)
+{
+    size_t b = 0;
+    const size_t nh = (size_t)ncfg * 4;
+    b += (nh * reps * 4 + 15) / 16 * 16;                       // counts
+    if (weighted) b += nh * 16;                                 // split fixed-point sums
+    b += ((size_t)(kMaxBins + 1) * 8);                          // ok_lo/ok_hi
+    b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;           // t64
+    b += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;             // config table (u16)
+    const size_t rec = fp64 ? 32 : 16;                          // coords + packed bin/class
+    b += (size_t)kmax * (rec + 4 + (weighted ? 8 : 0));         // + sorted index + weight
+    return b;
+}
+
+// record packing: bits 0..7 bin (0-based), bit 8 random
+template <bool FP64, bool W>
+__global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const BinDesc& B = *a.bins;
+    const int nb = B.nb;
+    const int ncfg = a.ncfg;
+    const int nh = ncfg * 4;
+    const int reps = a.reps;
+    const int kmax = a.kmax;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int rep = lane & (reps - 1);
+
+    unsigned char* p = smem;
+    uint32_t* hc = (uint32_t*)p;
+    p += ((size_t)nh * reps * 4 + 15) / 16 * 16;
+    unsigned long long* hw = (unsigned long long*)p;
+    if (W) p += (size_t)nh * 16;
+    float2* okb = (float2*)p;
+    p += (size_t)(kMaxBins + 1) * 8;
+    double* t64 = (double*)p;
+    p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
+    uint16_t* tab = (uint16_t*)p;
+    p += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;
+    float4* rec32 = (float4*)p;          // FP32: x, y, z relative to the apex, packed bin/class
+    double4* rec64 = (double4*)p;        // FP64: absolute x, y, z, packed bin/class (as a double)
+    p += (size_t)kmax * (FP64 ? 32 : 16);
+    int32_t* jl = (int32_t*)p;
+    p += (size_t)kmax * 4;
+    double* wl = (double*)p;             // neighbour weights * 2^shift3
+
+    __shared__ int s_item, s_k, s_over;
+
+    for (int k = tid; k < nh * reps; k += kWG) hc[k] = 0;
+    if (W)
+        for (int k = tid; k < 2 * nh; k += kWG) hw[k] = 0;
+    for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
+    for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
+    for (int k = tid; k < nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
+    __syncthreads();
+
+    const float ua = B.ua, ub = B.ub;
+    const bool logb = B.log_bins != 0;
+    const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
+    const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
+    const int shift = B.fx_shift3;
+    const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
+    const double inv_h = 1.0 / a.g.h;
+
+    unsigned long long units = 0, binned = 0, rechk = 0, over = 0, cand = 0;
+    unsigned long long bound = 0;   // u32 increments since the last drain (uniform)
+    const int xcd = blockIdx.x & 7;
+    int tried = 0;
+
+    while (true) {
+        if (tid == 0) {
+            int it = -1;
+            while (tried < 8) {
+                const int sidx = (xcd + tried) & 7;
+                const int lo = (int)(((long long)a.n_items * sidx) >> 3);
+                const int hi = (int)(((long long)a.n_items * (sidx + 1)) >> 3);
+                const int k = atomicAdd(&a.queue[sidx], 1);
+                if (lo + k < hi) { it = lo + k; break; }
+                ++tried;
+            }
+            s_item = it;
+            s_k = 0;
+            s_over = 0;
+        }
+        __syncthreads();
+        const int apex = s_item;
+        if (apex < 0) break;
+        const uint32_t fa = a.flags[apex];
+        if (fa & 2u) {          // buffer point: never an apex (gramsci.F90:476)
+            __syncthreads();
+            continue;
+        }
+        const double4 Pa = a.pw[apex];
+        const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_h), 0), nx - 1);
+        const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), ny - 1);
+        const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nz - 1);
+
+        // ---- 1. gather the kept neighbours with their exact bin
+        for (int r = 0; r < a.nrows; ++r) {
+            const Row rw = a.rows[r];
+            const int by = cy + rw.dy, bz = cz + rw.dz;
+            if (by < 0 || by >= ny || bz < 0 || bz >= nz) continue;
+            const int bx0 = max(cx + rw.x0, 0), bx1 = min(cx + rw.x1, nx - 1);
+            if (bx0 > bx1) continue;
+            const int rb = (bz * ny + by) * nx;
+            const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
+            for (int j = js + tid; j < je; j += kWG) {
+                ++cand;
+                const double4 Pj = a.pw[j];
+                const double s = exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z);
+                if (!(s > t_lo && s < t_hi)) continue;
+                const float sf = (float)s;
+                int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub));
+                b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                while (b > 0 && s < t64[b]) --b;
+                while (b < nb && s >= t64[b + 1]) ++b;
+                if (b == nb) { ++over; continue; }
+                const int pos = atomicAdd(&s_k, 1);
+                if (pos >= kmax) { s_over = 1; continue; }
+                const uint32_t fj = a.flags[j];
+                const uint32_t pk = (uint32_t)b | ((fj & 1u) << 8);
+                if constexpr (FP64) {
+                    rec64[pos] = make_double4(Pj.x, Pj.y, Pj.z, (double)pk);
+                } else {
+                    rec32[pos] = make_float4((float)(Pj.x - Pa.x), (float)(Pj.y - Pa.y), (float)(Pj.z - Pa.z),
+                                             __uint_as_float(pk));
+                }
+                jl[pos] = j;
+                if constexpr (W) wl[pos] = ldexp(Pj.w, shift);
+            }
+        }
+        __syncthreads();
+        const int K = min(s_k, kmax);
+        if (s_over) {
+            if (tid == 0) atomicExch(a.err, 1);
+        }
+        const long long P = (long long)K * (K - 1) / 2;
+        if (tid == 0) units += (unsigned long long)P;
+        const int ra = (int)(fa & 1u);
+        const double wa = Pa.w;
+
+        // ---- 2. neighbour pairs
+        // flat pair index pi -> (u, v), u < v, row u holding v = u+1 .. K-1
+        int u = 0;
+        long long rem = tid;
+        while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
+        for (long long pi = tid; pi < P; pi += kWG) {
+            const int v = u + 1 + (int)rem;
+            uint32_t pu, pv;
+            int b3 = -1;
+            if constexpr (FP64) {
+                const double4 Ru = rec64[u], Rv = rec64[v];
+                pu = (uint32_t)Ru.w;
+                pv = (uint32_t)Rv.w;
+                const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
+                if (s > t_lo && s < t_hi) {
+                    const float sf = (float)s;
+                    int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub));
+                    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                    while (b > 0 && s < t64[b]) --b;
+                    while (b < nb && s >= t64[b + 1]) ++b;
+                    if (b == nb) ++over; else b3 = b;
+                }
+            } else {
+                const float4 Ru = rec32[u], Rv = rec32[v];
+                pu = __float_as_uint(Ru.w);
+                pv = __float_as_uint(Rv.w);
+                const float dx = Rv.x - Ru.x, dy = Rv.y - Ru.y, dz = Rv.z - Ru.z;
+                const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                if (s > rej_lo && s < rej_hi) {
+                    int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(s), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(s), ua, ub));
+                    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                    const float2 ok = okb[b];
+                    if (s >= ok.x && s < ok.y) {
+                        b3 = b;
+                    } else {
+                        ++rechk;
+                        const double4 Pu = a.pw[jl[u]], Pv = a.pw[jl[v]];
+                        const double se = exact_s(Pu.x, Pu.y, Pu.z, Pv.x, Pv.y, Pv.z);
+                        if (se > t_lo && se < t_hi) {
+                            const int be = exact_bin(se, t64, nb);
+                            if (be == nb) ++over; else b3 = be;
+                        }
+                    }
+                }
+            }
+            if (b3 >= 0) {
+                const int cfg = tab[((pu & 255u) * nb + (pv & 255u)) * nb + b3];
+                const int cls = ra + (int)((pu >> 8) & 1u) + (int)((pv >> 8) & 1u);
+                const int hk = cfg * 4 + cls;
+                atomicAdd(&hc[hk * reps + rep], 1u);
+                ++binned;
+                if constexpr (W) {
+                    uint32_t qlo, qhi;
+                    quantise((wa * ldexp(wl[u], -shift)) * wl[v], qlo, qhi);
+                    atomicAdd(&hw[2 * hk], (unsigned long long)qlo);
+                    atomicAdd(&hw[2 * hk + 1], (unsigned long long)qhi);
+                }
+            }
+            rem += kWG;
+            while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
+        }
+        bound += (unsigned long long)P;
+        __syncthreads();
+        if (bound > (1ull << 31)) {
+            for (int k = tid; k < nh; k += kWG) {
+                unsigned long long s = 0;
+                for (int r = 0; r < reps; ++r) { s += hc[k * reps + r]; hc[k * reps + r] = 0; }
+                if (s) atomicAdd(&a.out_cnt[k], s);
+                if (W) {
+                    const unsigned long long lo = hw[2 * k], hi = hw[2 * k + 1];
+                    hw[2 * k] = 0; hw[2 * k + 1] = 0;
+                    if (lo | hi) {
+                        const unsigned long long L = (hi << 32) + lo;
+                        const unsigned long long H = (hi >> 32) + (L < lo ? 1ull : 0ull);
+                        glb_add128(&a.out_w[2 * k], L, H);
+                    }
+                }
+            }
+            bound = 0;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (int k = tid; k < nh; k += kWG) {
+        unsigned long long s = 0;
+        for (int r = 0; r < reps; ++r) s += hc[k * reps + r];
+        if (s) atomicAdd(&a.out_cnt[k], s);
+        if (W) {
+            const unsigned long long lo = hw[2 * k], hi = hw[2 * k + 1];
+            if (lo | hi) {
+                const unsigned long long L = (hi << 32) + lo;
+                const unsigned long long H = (hi >> 32) + (L < lo ? 1ull : 0ull);
+                glb_add128(&a.out_w[2 * k], L, H);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        units += __shfl_down(units, off, 64);
+        binned += __shfl_down(binned, off, 64);
+        rechk += __shfl_down(rechk, off, 64);
+        over += __shfl_down(over, off, 64);
+        cand += __shfl_down(cand, off, 64);
+    }
+    if (lane == 0) {
+        if (cand) atomicAdd(&a.out_stats[0], cand);
+        if (rechk) atomicAdd(&a.out_stats[1], rechk);
+        if (over) atomicAdd(&a.out_stats[2], over);
+        if (units) atomicAdd(&a.out_stats[3], units);
+        if (binned) atomicAdd(&a.out_stats[4], binned);
+    }
+}
+
+template <bool FP64, bool W>
+static hipError_t launch_t3(const TripArgs& a, size_t lds, int nblocks, hipStream_t st)
+{
+    auto kern = k_triplet3<FP64, W>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kWG), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st)
+{
+    const size_t lds = triplet3_lds_bytes(nb, a.ncfg, a.reps, a.kmax, fp64, weighted);
+    if (fp64) return weighted ? launch_t3<true, true>(a, lds, nblocks, st) : launch_t3<true, false>(a, lds, nblocks, st);
+    return weighted ? launch_t3<false, true>(a, lds, nblocks, st) : launch_t3<false, false>(a, lds, nblocks, st);
+}
+
+template <bool FP64, bool W>
+static int occ3(size_t lds)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_triplet3<FP64, W>, kWG, lds) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+int triplet3_occupancy(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted)
+{
+    const size_t lds = triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted);
+    if (fp64) return weighted ? occ3<true, true>(lds) : occ3<true, false>(lds);
+    return weighted ? occ3<false, true>(lds) : occ3<false, false>(lds);
+}
+
+}  // namespace gs

@@ -1,0 +1,181 @@
+"""3PCF and 4PCF parity of the HIP path against the oracle and the
+reference's output files.
+
+Bar: triplet / quad counts per configuration and class bit-exact (both
+precisions for the 3PCF; the 4PCF is evaluated in FP64); N2/N3 (and the disc
+sums) within 1e-12 relative of the oracle, with an absolute floor of
+1e-12 * max|N| for cells where a signed sum cancels.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from golden_util import close_e147, load_input, manifest, parse_rows, read_text
+from gramsci_amd import catalog as C
+from gramsci_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+MAN = manifest()
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def _cat(name):
+    d, r, wd, wr = load_input(name)
+    xyz = np.concatenate([d, r]) if len(r) else d
+    w = None
+    if wd is not None:
+        w = np.concatenate([wd, wr if wr is not None else np.ones(len(r))])
+    return xyz, w, len(d)
+
+
+def _close(a, b, rel=REL):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    fin = np.isfinite(b)
+    floor = rel * max(np.abs(b[fin]).max() if fin.any() else 0.0, 1e-300)
+    return np.all(np.abs(a[fin] - b[fin]) <= np.maximum(rel * np.abs(b[fin]), floor))
+
+
+def check3(eng, xyz, w, nd, p, buf=None, precisions=("fp64", "fp32")):
+    ref = O.pcf3(xyz, w, nd, p, buf=buf)
+    outs = []
+    for prec in precisions:
+        eng.set_catalogue(xyz, w, nd, buf)
+        r = eng.pcf3(p.rmin, p.rmax, p.nbins, p.log_bins, prec)
+        print(f"  3pcf {prec}: units={r['stats']['n_units']} binned={r['stats']['n_binned']} "
+              f"recheck={r['stats']['n_recheck']}", flush=True)
+        assert np.array_equal(r["cnt"], ref["cnt"]), (prec, np.argwhere(r["cnt"] != ref["cnt"])[:5])
+        assert r["stats"]["n_units"] == ref["n_cand"]
+        assert _close(r["n2"], ref["n2"]) and _close(r["n3"], ref["n3"])
+        outs.append(r)
+    return ref, outs
+
+
+def check4(eng, xyz, w, nd, p, buf=None):
+    ref = O.pcf4(xyz, w, nd, p, buf=buf)
+    eng.set_catalogue(xyz, w, nd, buf)
+    r = eng.pcf4(p.rmin, p.rmax, p.nbins, p.log_bins)
+    print(f"  4pcf: units={r['stats']['n_units']} binned={r['stats']['n_binned']}", flush=True)
+    assert np.array_equal(r["cnt"], ref["cnt"])
+    assert r["stats"]["n_units"] == ref["n_quads"]
+    assert _close(r["n2"], ref["n2"]) and _close(r["n3"], ref["n3"])
+    assert _close(r["disc_n"], ref["disc_n"]) and _close(r["disc_r"], ref["disc_r"])
+    return ref, r
+
+
+@pytest.mark.parametrize("name", [n for n, c in MAN.items() if c["mode"] == "-3pcf"])
+def test_3pcf_golden(eng, name):
+    c = MAN[name]
+    p = O.Params(**c["params"])
+    xyz, w, nd = _cat(name)
+    print(f"case {name}", flush=True)
+    ref, outs = check3(eng, xyz, w, nd, p)
+    rows = parse_rows(read_text(name, "DR" if "DR" in c["runs"] else "D"))
+    for r in outs:
+        assert close_e147(rows[:, 6], r["n2"]).all()
+        assert close_e147(rows[:, 7], r["n3"]).all()
+
+
+@pytest.mark.parametrize("name", [n for n, c in MAN.items() if c["mode"] == "-4pcf"])
+def test_4pcf_golden(eng, name):
+    c = MAN[name]
+    p = O.Params(**c["params"])
+    xyz, w, nd = _cat(name)
+    print(f"case {name}", flush=True)
+    ref, r = check4(eng, xyz, w, nd, p)
+    rows = parse_rows(read_text(name, "DR"))
+    assert close_e147(rows[:, 8], r["n2"].reshape(-1)).all()
+    assert close_e147(rows[:, 9], r["n3"].reshape(-1)).all()
+
+
+def test_3pcf_exact_counts_vs_reference_runs(eng):
+    name = "p3"
+    p = O.Params(**MAN[name]["params"])
+    d, r, _, _ = load_input(name)
+    for run, cat in (("D", d), ("R", r), ("T", np.concatenate([d, r]))):
+        eng.set_catalogue(cat, None, len(cat))
+        out = eng.pcf3(p.rmin, p.rmax, p.nbins, False, "fp32")
+        rows = parse_rows(read_text(name, run))
+        got = np.rint(rows[:, 6] * float(len(cat)) ** 3).astype(np.int64)
+        assert np.array_equal(out["cnt"][:, 0].astype(np.int64), got), run
+
+
+def test_4pcf_exact_counts_vs_reference_runs(eng):
+    name = "p4"
+    p = O.Params(**MAN[name]["params"])
+    d, r, _, _ = load_input(name)
+    for run, cat in (("D", d), ("R", r), ("T", np.concatenate([d, r]))):
+        eng.set_catalogue(cat, None, len(cat))
+        out = eng.pcf4(p.rmin, p.rmax, p.nbins)
+        rows = parse_rows(read_text(name, run))
+        got = np.rint(rows[:, 8] * float(len(cat)) ** 4).astype(np.int64)
+        assert np.array_equal(out["cnt"][..., 0].reshape(-1).astype(np.int64), got), run
+
+
+def test_3pcf_clustered_log_weighted_buffer(eng):
+    rng = np.random.default_rng(7)
+    d = C.clustered(6000, 120.0, 71)
+    r = C.uniform(12000, 120.0, 72)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(6000, 0.2, 73), C.lognormal_weights(12000, 0.1, 74)])
+    buf = (rng.random(len(xyz)) < 0.25).astype(np.int32)
+    check3(eng, xyz, w, 6000, O.Params(2.0, 14.0, 6, 1, True), buf=buf)
+    check3(eng, xyz, None, 6000, O.Params(3.0, 12.0, 4, 1, False))
+
+
+def test_3pcf_config3_density_slab(eng):
+    # a 150^3 slab at the density of the 3PCF configuration (600k points in 500^3)
+    d = C.clustered(5400, 150.0, 81)
+    r = C.uniform(10800, 150.0, 82)
+    xyz = np.concatenate([d, r])
+    ref, outs = check3(eng, xyz, None, 5400, O.Params(10.0, 30.0, 5, 1, False))
+    assert ref["n_cand"] > 1e8
+
+
+def test_4pcf_weighted_buffer_and_order(eng):
+    rng = np.random.default_rng(9)
+    d = C.clustered(3000, 70.0, 91)
+    r = C.uniform(3000, 70.0, 92)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(3000, 0.2, 93), np.ones(3000)])
+    buf = (rng.random(len(xyz)) < 0.2).astype(np.int32)
+    check4(eng, xyz, w, 3000, O.Params(3.0, 9.0, 3, 1, False), buf=buf)
+    # the result depends on the input order (ascending original index), as in the reference
+    perm = np.concatenate([rng.permutation(3000), 3000 + rng.permutation(3000)])
+    check4(eng, xyz[perm], None, 3000, O.Params(3.0, 9.0, 3, 1, False))
+
+
+def test_4pcf_lattice_and_log(eng):
+    lat = C.lattice(9, 1.0, 0.0)
+    check4(eng, lat, None, len(lat), O.Params(0.5, 2.5, 2, 1, False))
+    d = C.clustered(2500, 50.0, 95)
+    check4(eng, d, None, len(d), O.Params(1.0, 8.0, 4, 1, True))
+
+
+def test_3pcf_rejects_rsd_and_large_neighbour_sets(eng):
+    from gramsci_amd._lib import GsError
+    d = C.uniform(2000, 50.0, 1)
+    eng.set_catalogue(d, None, len(d))
+    with pytest.raises(GsError):
+        eng.pcf2(1.0, 5.0, 3, 1, False, "fp32")  # fine
+        raise GsError(-1, "sentinel")
+    p = Engine.params(1.0, 5.0, 3, nmu=4)
+    import ctypes
+    from gramsci_amd import _lib as L
+    cnt = np.zeros((10, 4), np.uint64)
+    rc = L.lib().gs_count_3pcf(eng._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), None, None, None, None)
+    assert rc == L.GS_EINVAL
+    # every point within rmax of every other: more neighbours than the LDS list holds
+    dense = C.uniform(9000, 10.0, 2)
+    eng.set_catalogue(dense, None, len(dense))
+    with pytest.raises(GsError) as e:
+        eng.pcf3(0.1, 30.0, 3)
+    assert e.value.code == L.GS_ELIMIT

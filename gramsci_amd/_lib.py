@@ -82,6 +82,11 @@ def lib():
     L.gs_pcf3.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, sp]
     L.gs_pcf4.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, dp, sp]
     L.gs_bin_thresholds.argtypes = [pp, dp]
+    L.gs_adjoint_2pcf.argtypes = [vp, pp, dp, dp, dp, sp]
+    i64p = P(ctypes.c_int64)
+    L.gs_set_catalogue_batched.argtypes = [vp, dp, dp, i32p, i64p, i64p, ctypes.c_int]
+    L.gs_count_2pcf_batched.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
+    L.gs_adjoint_2pcf_device.argtypes = [vp, pp, dp, dp, vp, sp]
     for name in exported_names():
         getattr(L, name)  # every declared symbol must resolve
         if name not in ("gs_last_error", "gs_version"):

@@ -54,7 +54,7 @@ def grf_templates(n_fields: int, grid: int, box: float, corr_len: float, seed: i
     out = np.empty((n_fields, grid, grid, grid))
     for f in range(n_fields):
         white = rng.normal(size=(grid, grid, grid))
-        field = np.fft.irfftn(np.fft.rfftn(white) * amp, s=(grid, grid, grid))
+        field = np.fft.irfftn(np.fft.rfftn(white) * amp, s=(grid, grid, grid), axes=(0, 1, 2))
         field -= field.mean()
         field /= field.std()
         out[f] = field
@@ -89,3 +89,34 @@ def lattice(n_side: int, spacing: float = 1.0, origin: float = 0.0) -> np.ndarra
     a = origin + spacing * np.arange(n_side, dtype=np.float64)
     x, y, z = np.meshgrid(a, a, a, indexing="ij")
     return np.column_stack([x.ravel(), y.ravel(), z.ravel()])
+
+
+def regions(n_regions: int, seed: int, rmax: float, median: float = 4e4, sigma: float = 1.0,
+            n_min: float = 5e3, n_max: float = 4e5, density: float = 5e-4, ran_factor: int = 2):
+    """Domain-decomposed sub-volumes of uneven size.  Region r has N_r data
+    points (lognormal around `median`, clipped to [n_min, n_max]) in a cube of
+    side (N_r/density)^(1/3), clustered by the recipe of clustered(), plus
+    ran_factor*N_r uniform randoms, and an rmax-wide shell around the cube at
+    the same densities whose points are flagged as buffer (never an apex).
+    Returns xyz (n,3), buf (n,) int32, off (n_regions+1,) int64 and
+    ndata (n_regions,) int64; region r is xyz[off[r]:off[r+1]], data first."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.exp(rng.normal(np.log(median), sigma, n_regions)), n_min, n_max).astype(np.int64)
+    xs, bufs, off, ndata = [], [], [0], []
+    for r, nr in enumerate(sizes):
+        side = (nr / density) ** (1.0 / 3.0)
+        big = side + 2.0 * rmax
+        f = (big / side) ** 3
+        nd = int(round(nr * f))
+        d = clustered(nd, big, seed * 1000 + 2 * r) - rmax
+        ran = uniform(int(round(ran_factor * nr * f)), big, seed * 1000 + 2 * r + 1) - rmax
+        pts = np.concatenate([d, ran])
+        inside = np.all((pts >= 0.0) & (pts < side), axis=1)
+        # place the regions side by side (they never interact)
+        pts = pts + np.array([r * (big + 10.0), 0.0, 0.0])
+        xs.append(pts)
+        bufs.append((~inside).astype(np.int32))
+        ndata.append(len(d))
+        off.append(off[-1] + len(pts))
+    return (np.concatenate(xs), np.concatenate(bufs), np.array(off, dtype=np.int64),
+            np.array(ndata, dtype=np.int64))

@@ -70,6 +70,7 @@ template struct DevBuf<double4>;
 template struct DevBuf<float4>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<unsigned char>;
+template struct DevBuf<int64_t>;
 
 // ------------------------------------------------------------ thresholds
 // The reference's radial bin of a squared distance s (gramsci.F90:136-145,
@@ -323,6 +324,7 @@ struct gs_ctx {
     DevBuf<int32_t> queue;
     DevBuf<double> part;
     GsExtra extra;
+    GsBatch batch;
 };
 
 using namespace gs;
@@ -375,6 +377,7 @@ extern "C" int gs_destroy(gs_ctx* c)
     c->orig.release();
     c->params.release(); c->out.release(); c->queue.release(); c->part.release();
     extra_release(c->extra);
+    batch_release(c->batch);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -491,6 +494,14 @@ extern "C" int gs_set_catalogue_device(gs_ctx* c, const double* d_xyz, const dou
 }
 
 namespace gs {
+
+// h rounded up to 11 significant bits: then k*h is exact in FP32 for |k| < 2^13
+double round11(double h)
+{
+    int ex;
+    const double fr = std::frexp(h, &ex);
+    return std::ldexp(std::ceil(std::ldexp(fr, 11)), ex - 11);
+}
 
 // Picks the cell edge: rmax / m, widened if the grid would exceed kMaxCells.
 static double choose_h(const gs_ctx* c, double rmax, int m)
@@ -762,8 +773,14 @@ CtxView view(gs_ctx* c)
     v.flags = c->flags.ptr;
     v.orig = c->orig.ptr;
     v.cell_start = c->cell_start.ptr;
+    v.pl = c->pl.ptr;
+    v.row_chunk = c->chunk_off.ptr;
+    v.w = c->has_w ? c->w.ptr : nullptr;
     v.n_chunks = c->n_chunks;
     v.extra = &c->extra;
+    v.batch = &c->batch;
+    v.xyz = c->xyz.ptr;
+    v.buf = c->has_buf ? c->buf.ptr : nullptr;
     return v;
 }
 int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out)
@@ -779,4 +796,26 @@ double ctx_xmax(gs_ctx* c)
     return xmax;
 }
 int ctx_set_device(gs_ctx* c) { return set_device(c); }
+int ctx_upload(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf, int64_t n)
+{
+    int rc;
+    c->n_data = n;
+    c->n_rand = 0;
+    c->has_w = w != nullptr;
+    c->has_buf = buf != nullptr;
+    c->grid_valid = false;
+    if ((rc = c->xyz.reserve(3 * n))) return rc;
+    hipError_t e = hipMemcpyAsync(c->xyz.ptr, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && w) {
+        if ((rc = c->w.reserve(n))) return rc;
+        e = hipMemcpyAsync(c->w.ptr, w, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    }
+    if (e == hipSuccess && buf) {
+        if ((rc = c->buf.reserve(n))) return rc;
+        e = hipMemcpyAsync(c->buf.ptr, buf, n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "catalogue upload");
+    if ((rc = host_weight_stats(c, w))) return rc;
+    return finish_catalogue(c);
+}
 }  // namespace gs

@@ -1,0 +1,355 @@
+// batched.cpp -- many independent regions in one persistent launch.
+//
+// The reference's scaling route is offline domain decomposition: the volume
+// is cut into regions with an rmax-wide buffer shell (domain_decomposition.f90
+// :49-74), each region is run separately in -cut mode with the shell points
+// flagged as never-apex (gramsci.F90:1224-1231, :363), and the per-region
+// outputs are summed afterwards.  Here all regions are counted by ONE launch
+// of the persistent pair kernel: every region keeps its own grid, points and
+// output histogram, the work queue spans the x-row chunks of all regions (so
+// uneven regions balance automatically), and the whole pipeline - grid build
+// (counting sort + scans) and the pair kernel - is captured once in a
+// hipGraph and replayed.  Buffers are cleared by fill kernels (no memset
+// nodes in the graph).  The cells-per-rmax factor is chosen by timing the
+// captured graph for each candidate on the actual batch (cell_div = 0), and
+// the choice is cached per parameter set.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gs_api_internal.h"
+#include "gs_kernels.h"
+
+using namespace gs;
+
+namespace gs {
+
+void batch_release(GsBatch& b)
+{
+    for (auto& pl : b.plans) {
+        if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+        if (pl.graph) (void)hipGraphDestroy(pl.graph);
+        pl.blob.release();
+    }
+    b.plans.clear();
+    b.d_off.release(); b.d_row_base.release(); b.cell_of.release(); b.count.release(); b.cell_start.release();
+    b.nch.release(); b.row_chunk.release(); b.scan_tmp.release(); b.queue.release(); b.orig.release();
+    b.d_regions.release(); b.params.release(); b.pw.release(); b.pl.release(); b.flags.release(); b.out.release();
+    b.valid = false;
+}
+
+static std::string plan_key(const gs_params& p)
+{
+    char k[160];
+    snprintf(k, sizeof(k), "%.17g/%.17g/%d/%d/%d/%d", p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, p.precision);
+    return k;
+}
+
+static bool same_params(const gs_params& a, const gs_params& b)
+{
+    return a.rmin == b.rmin && a.rmax == b.rmax && a.nbins == b.nbins && a.nmu == b.nmu &&
+           a.log_bins == b.log_bins && a.precision == b.precision;
+}
+
+// Builds (or finds) the captured pipeline for parameters p at m cells per rmax.
+static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
+{
+    CtxView v = view(c);
+    GsBatch& B = *v.batch;
+    for (auto& pl : B.plans)
+        if (pl.m == m && same_params(pl.p, p)) {
+            *out = &pl;
+            return GS_OK;
+        }
+    const int nreg = B.nreg;
+    const int64_t n = v.n_data;   // every point of the batch is held as "data" in the context buffers
+    const int nb = p.nbins, nm = p.nmu;
+    const bool rsd = nm >= 2, fp64 = p.precision == GS_FP64, weighted = v.has_w;
+    const int nbt = 3 * nb * nm;
+
+    // cell edge and per-region grids
+    const double h = round11(p.rmax / m);
+    std::vector<RegionDesc> reg(nreg);
+    std::vector<int32_t> row_base(nreg + 1, 0);
+    int64_t cells = 0, rows_tot = 0;
+    for (int r = 0; r < nreg; ++r) {
+        RegionDesc& d = reg[r];
+        memset(&d, 0, sizeof(d));
+        int64_t nc = 1;
+        for (int k = 0; k < 3; ++k) {
+            d.lo[k] = B.bbox[6 * r + k];
+            d.dim[k] = (int32_t)std::floor((B.bbox[6 * r + 3 + k] - B.bbox[6 * r + k]) / h) + 1;
+            nc *= d.dim[k];
+        }
+        d.cell_base = (int32_t)cells;
+        d.row_base = (int32_t)rows_tot;
+        d.pt_base = (int32_t)B.off[r];
+        d.n_data = (int32_t)B.ndata[r];
+        row_base[r] = (int32_t)rows_tot;
+        cells += nc;
+        rows_tot += (int64_t)d.dim[1] * d.dim[2];
+        if (cells > (1 << 26)) return fail(GS_ELIMIT, "batched grid too large (%lld cells)", (long long)cells);
+    }
+    row_base[nreg] = (int32_t)rows_tot;
+    const int mst = (int)std::ceil(p.rmax / h - 1e-12);
+    std::vector<Row> rows = half_stencil(h, p.rmax, mst);
+    BinDesc bins;
+    int rc;
+    if ((rc = fill_bins(p, (mst + 1) * h, B.xmax, B.wmax, bins))) return rc;
+
+    const size_t nout = (size_t)nreg * 3 * nbt + 4;
+    const int64_t scan_n = std::max<int64_t>(cells + 1, rows_tot + 1);
+    // captured graphs hold raw pointers to the shared work buffers: if any of
+    // them has to grow, every existing plan is destroyed before reallocating
+    const bool grow = (size_t)n > B.cell_of.cap || (size_t)(cells + 1) > B.count.cap ||
+                      (size_t)(cells + 1) > B.cell_start.cap || (size_t)(rows_tot + 1) > B.nch.cap ||
+                      (size_t)(rows_tot + 1) > B.row_chunk.cap || (size_t)scan_scratch_size(scan_n) > B.scan_tmp.cap ||
+                      nout > B.out.cap;
+    if (grow) {
+        for (auto& pl : B.plans) {
+            if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+            if (pl.graph) (void)hipGraphDestroy(pl.graph);
+            pl.blob.release();
+        }
+        B.plans.clear();
+    }
+    if ((rc = B.cell_of.reserve(n)) || (rc = B.count.reserve(cells + 1)) || (rc = B.cell_start.reserve(cells + 1)) ||
+        (rc = B.nch.reserve(rows_tot + 1)) || (rc = B.row_chunk.reserve(rows_tot + 1)) ||
+        (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(8)) ||
+        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n)) || (rc = B.flags.reserve(n)) ||
+        (rc = B.out.reserve(nout)))
+        return rc;
+    // per-plan parameter blob: regions, bins, stencil rows (kept for the plan's lifetime)
+    const size_t reg_bytes = ((nreg * sizeof(RegionDesc) + 15) / 16) * 16;
+    const size_t bin_off = reg_bytes, rows_off = bin_off + ((sizeof(BinDesc) + 15) / 16) * 16;
+    const size_t rb_off = rows_off + ((rows.size() * sizeof(Row) + 15) / 16) * 16;
+    const size_t blob = rb_off + (nreg + 1) * sizeof(int32_t) + 16;
+    // each plan owns its blob so that graphs of several parameter sets can coexist
+    B.plans.emplace_back();
+    GsBatch::Plan& P = B.plans.back();
+    DevBuf<unsigned char>* pblob = &P.blob;
+    if ((rc = pblob->reserve(blob))) { B.plans.pop_back(); return rc; }
+    hipError_t e = hipMemcpyAsync(pblob->ptr, reg.data(), nreg * sizeof(RegionDesc), hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pblob->ptr + bin_off, &bins, sizeof(bins), hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess && !rows.empty())
+        e = hipMemcpyAsync(pblob->ptr + rows_off, rows.data(), rows.size() * sizeof(Row), hipMemcpyHostToDevice,
+                           v.stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pblob->ptr + rb_off, row_base.data(), (nreg + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                           v.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+    if (e != hipSuccess) { pblob->release(); B.plans.pop_back(); return hip_fail(e, "batch upload"); }
+    const RegionDesc* d_reg = (const RegionDesc*)pblob->ptr;
+
+    PairArgs a{};
+    a.pw = B.pw.ptr;
+    a.pl = B.pl.ptr;
+    a.flags = B.flags.ptr;
+    a.cell_start = B.cell_start.ptr;
+    a.row_chunk = B.row_chunk.ptr;
+    a.rows = (const Row*)(pblob->ptr + rows_off);
+    a.bins = (const BinDesc*)(pblob->ptr + bin_off);
+    a.queue = B.queue.ptr;
+    a.out_cnt = B.out.ptr;
+    a.out_w = B.out.ptr + (size_t)nreg * nbt;
+    a.out_stats = B.out.ptr + (size_t)nreg * 3 * nbt;
+    a.g.h = h;
+    a.g.h32 = (float)h;
+    a.g.m = mst;
+    a.nrows = (int)rows.size();
+    a.reps = 1;
+    a.mirror = (rsd && !v.has_buf) ? 1 : 0;
+    a.regions = d_reg;
+    a.row_base = (const int32_t*)(pblob->ptr + rb_off);
+    a.n_chunks_dev = B.row_chunk.ptr + rows_tot;
+    a.nreg = nreg;
+    if (pair2_lds_bytes(nb, rsd ? nm : 1, 1, fp64, weighted, true) > 64 * 1024) {
+        pblob->release(); B.plans.pop_back();
+        return fail(GS_ELIMIT, "per-wave histograms of %d bins do not fit in LDS", nbt);
+    }
+    const int occ = pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, true);
+    const int nblocks = v.n_cu * occ;
+
+    // capture: clear, count sort into cells, scans, scatter, row chunks, pair kernel
+    e = hipStreamBeginCapture(v.stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
+    if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 8, 0, v.stream);
+    if (e == hipSuccess) e = launch_fill_u32(B.out.ptr, (int64_t)nout * 2, 0, v.stream);
+    if (e == hipSuccess)
+        e = launch_cell_count_b(v.xyz, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
+    if (e == hipSuccess) e = exclusive_scan(B.count.ptr, B.cell_start.ptr, cells + 1, B.scan_tmp.ptr, v.stream);
+    if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
+    if (e == hipSuccess)
+        e = launch_scatter_b(v.xyz, v.w, v.buf, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.cell_start.ptr,
+                             B.count.ptr, B.pw.ptr, B.pl.ptr, B.flags.ptr, B.orig.ptr, v.stream);
+    if (e == hipSuccess)
+        e = launch_row_chunks_b(B.cell_start.ptr, d_reg, a.row_base, nreg, (int)rows_tot, B.nch.ptr, v.stream);
+    if (e == hipSuccess) e = exclusive_scan(B.nch.ptr, B.row_chunk.ptr, rows_tot + 1, B.scan_tmp.ptr, v.stream);
+    if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, nblocks, v.stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(v.stream, &graph);
+    if (e == hipSuccess) e = e2;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        pblob->release(); B.plans.pop_back();
+        return hip_fail(e, "batched graph capture");
+    }
+    P.p = p;
+    P.m = m;
+    P.graph = graph;
+    P.exec = exec;
+    P.nbt = nbt;
+    P.fx_shift = bins.fx_shift2;
+    P.mst = mst;
+    *out = &P;
+    return GS_OK;
+}
+
+static int run_plan(gs_ctx* c, GsBatch::Plan& P, float* ms)
+{
+    CtxView v = view(c);
+    (void)hipEventRecord(v.ev[2], v.stream);
+    hipError_t e = hipGraphLaunch(P.exec, v.stream);
+    (void)hipEventRecord(v.ev[3], v.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+    if (e != hipSuccess) return hip_fail(e, "batched graph launch");
+    if (ms) (void)hipEventElapsedTime(ms, v.ev[2], v.ev[3]);
+    return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf,
+                                        const int64_t* off, const int64_t* ndata, int nreg)
+{
+    if (!c || !off || !ndata || nreg < 1 || nreg > 65536) return fail(GS_EINVAL, "bad batched catalogue arguments");
+    const int64_t n = off[nreg];
+    if (off[0] != 0 || n < 0 || n > (int64_t)INT32_MAX - 1024 || (!xyz && n > 0))
+        return fail(GS_EINVAL, "region offsets must start at 0 and be non-decreasing");
+    for (int r = 0; r < nreg; ++r)
+        if (off[r + 1] < off[r] || ndata[r] < 0 || ndata[r] > off[r + 1] - off[r])
+            return fail(GS_EINVAL, "bad region %d (offsets or data count)", r);
+    int rc = ctx_set_device(c);
+    if (rc) return rc;
+    CtxView v = view(c);
+    GsBatch& B = *v.batch;
+    batch_release(B);
+    B.nreg = nreg;
+    B.off.assign(off, off + nreg + 1);
+    B.ndata.assign(ndata, ndata + nreg);
+    B.bbox.assign((size_t)6 * nreg, 0.0);
+    B.wd.assign(nreg, 0.0L);
+    B.wr.assign(nreg, 0.0L);
+    B.wmax = 1.0;
+    B.xmax = 0.0;
+    double wm = 0.0;
+    for (int r = 0; r < nreg; ++r) {
+        double* bb = &B.bbox[6 * r];
+        for (int k = 0; k < 3; ++k) { bb[k] = 1e308; bb[3 + k] = -1e308; }
+        if (off[r + 1] == off[r])
+            for (int k = 0; k < 6; ++k) bb[k] = 0.0;
+        for (int64_t i = off[r]; i < off[r + 1]; ++i) {
+            for (int k = 0; k < 3; ++k) {
+                const double x = xyz[3 * i + k];
+                if (!std::isfinite(x)) return fail(GS_EINVAL, "non-finite coordinate at point %lld", (long long)i);
+                bb[k] = std::min(bb[k], x);
+                bb[3 + k] = std::max(bb[3 + k], x);
+                B.xmax = std::max(B.xmax, std::fabs(x));
+            }
+            const double wi = w ? w[i] : 1.0;
+            if (!(wi >= 0.0) || !std::isfinite(wi)) return fail(GS_EINVAL, "weights must be finite and >= 0");
+            wm = std::max(wm, wi);
+            if (i - off[r] < ndata[r]) B.wd[r] += wi; else B.wr[r] += wi;
+        }
+    }
+    if (w) B.wmax = wm > 0 ? wm : 1.0;
+    if ((rc = ctx_upload(c, xyz, w, buf, n))) return rc;
+    std::vector<int32_t> o32(nreg + 1);
+    for (int r = 0; r <= nreg; ++r) o32[r] = (int32_t)off[r];
+    if ((rc = B.d_off.reserve(nreg + 1))) return rc;
+    hipError_t e = hipMemcpy(B.d_off.ptr, o32.data(), (nreg + 1) * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "region offsets upload");
+    B.valid = true;
+    return GS_OK;
+}
+
+extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2,
+                                     double* n3, gs_stats* st)
+{
+    if (!c) return fail(GS_EINVAL, "NULL context");
+    int rc = check_params(p, 2);
+    if (rc) return rc;
+    if ((rc = ctx_set_device(c))) return rc;
+    CtxView v = view(c);
+    GsBatch& B = *v.batch;
+    if (!B.valid) return fail(GS_EINVAL, "no batched catalogue (gs_set_catalogue_batched)");
+    const int nreg = B.nreg, nb = p->nbins, nm = p->nmu, nbt = 3 * nb * nm;
+    const bool weighted = v.has_w;
+
+    int m = p->cell_div;
+    if (m == 0) {
+        const std::string key = plan_key(*p);
+        for (auto& kv : B.tuned)
+            if (kv.first == key) m = kv.second;
+        if (m == 0) {
+            // autotune: time the captured pipeline for each candidate on this batch
+            float best = 1e30f;
+            for (int cand = 2; cand <= 6; ++cand) {
+                GsBatch::Plan* P;
+                if ((rc = get_plan(c, *p, cand, &P))) return rc;
+                float t = 0;
+                if ((rc = run_plan(c, *P, nullptr)) || (rc = run_plan(c, *P, &t))) return rc;
+                if (t < best) { best = t; m = cand; }
+            }
+            B.tuned.emplace_back(key, m);
+        }
+    }
+    GsBatch::Plan* P;
+    if ((rc = get_plan(c, *p, m, &P))) return rc;
+    float ms = 0;
+    if ((rc = run_plan(c, *P, &ms))) return rc;
+    const size_t nout = (size_t)nreg * 3 * nbt + 4;
+    std::vector<unsigned long long> h(nout);
+    hipError_t e = hipMemcpy(h.data(), B.out.ptr, nout * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "batched result copy");
+    uint64_t total = 0;
+    for (int r = 0; r < nreg; ++r) {
+        const unsigned long long* hc = h.data() + (size_t)r * nbt;
+        const unsigned long long* hw = h.data() + (size_t)nreg * nbt + (size_t)r * 2 * nbt;
+        std::vector<long double> S(nbt);
+        for (int k = 0; k < nbt; ++k) {
+            if (cnt) cnt[(size_t)r * nbt + k] = hc[k];
+            total += hc[k];
+            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift) : (long double)hc[k];
+            if (wsum) wsum[(size_t)r * nbt + k] = (double)S[k];
+        }
+        const long double Wd = B.wd[r], Wr = B.wr[r];
+        for (int b = 0; b < nb * nm; ++b) {
+            const long double dd = S[b], dr = S[nb * nm + b], rr = S[2 * nb * nm + b];
+            long double v2 = 0, v3 = 0;
+            if (Wd > 0) v2 += dd / (Wd * Wd);
+            if (Wd > 0 && Wr > 0) v2 -= dr / (Wd * Wr);
+            if (Wr > 0) { v2 += rr / (Wr * Wr); v3 = rr / (Wr * Wr); }
+            if (n2) n2[(size_t)r * nb * nm + b] = (double)v2;
+            if (n3) n3[(size_t)r * nb * nm + b] = (double)v3;
+        }
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->ms_count = ms;
+        st->ms_kernel_avg = ms;
+        st->n_units = total;
+        st->n_binned = total;
+        const unsigned long long* s = h.data() + (size_t)nreg * 3 * nbt;
+        st->n_candidates = s[0];
+        st->n_recheck = s[1];
+        st->n_overflow = s[2];
+        st->cell_div = P->mst;
+    }
+    return GS_OK;
+}

@@ -210,5 +210,122 @@ hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- batched regions
+__device__ __forceinline__ int region_of(const int32_t* off, int nreg, int64_t i)
+{
+    int lo = 0, hi = nreg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_cell_count_b(const double* __restrict__ xyz, int64_t n,
+                                                      const int32_t* __restrict__ off, int nreg,
+                                                      const RegionDesc* __restrict__ reg, double h,
+                                                      int32_t* __restrict__ cell_of, int32_t* __restrict__ count)
+{
+    const double inv_h = 1.0 / h;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const RegionDesc R = reg[region_of(off, nreg, i)];
+        int cx = cell_coord(xyz[3 * i + 0], R.lo[0], inv_h, R.dim[0]);
+        int cy = cell_coord(xyz[3 * i + 1], R.lo[1], inv_h, R.dim[1]);
+        int cz = cell_coord(xyz[3 * i + 2], R.lo[2], inv_h, R.dim[2]);
+        int c = R.cell_base + (cz * R.dim[1] + cy) * R.dim[0] + cx;
+        cell_of[i] = c;
+        atomicAdd(&count[c], 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter_b(const double* __restrict__ xyz, const double* __restrict__ w,
+                                                   const int32_t* __restrict__ buf, int64_t n,
+                                                   const int32_t* __restrict__ off, int nreg,
+                                                   const RegionDesc* __restrict__ reg, double h,
+                                                   const int32_t* __restrict__ cell_of,
+                                                   const int32_t* __restrict__ start, int32_t* __restrict__ fill,
+                                                   double4* __restrict__ pw, float4* __restrict__ pl,
+                                                   uint32_t* __restrict__ flags, int32_t* __restrict__ orig)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = region_of(off, nreg, i);
+        const RegionDesc R = reg[r];
+        const int c = cell_of[i];
+        const int pos = start[c] + atomicAdd(&fill[c], 1);
+        const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        pw[pos] = make_double4(x, y, z, w ? w[i] : 1.0);
+        const int lc = c - R.cell_base;
+        const int cx = lc % R.dim[0], cy = (lc / R.dim[0]) % R.dim[1], cz = lc / (R.dim[0] * R.dim[1]);
+        pl[pos] = make_float4((float)(x - (R.lo[0] + cx * h)), (float)(y - (R.lo[1] + cy * h)),
+                              (float)(z - (R.lo[2] + cz * h)), (float)cx);
+        const uint32_t f = ((i - R.pt_base) >= R.n_data ? 1u : 0u) | ((buf && buf[i] == 1) ? 2u : 0u);
+        flags[pos] = f;
+        orig[pos] = (int32_t)i;
+    }
+}
+
+// chunks of each global x-row; nch[total_rows] = 0
+__global__ __launch_bounds__(256) void k_row_chunks_b(const int32_t* __restrict__ start,
+                                                      const RegionDesc* __restrict__ reg,
+                                                      const int32_t* __restrict__ row_base, int nreg, int nrow_tot,
+                                                      int32_t* __restrict__ nch)
+{
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r <= nrow_tot; r += gridDim.x * blockDim.x) {
+        int v = 0;
+        if (r < nrow_tot) {
+            const RegionDesc R = reg[region_of(row_base, nreg, r)];
+            const int lr = r - R.row_base;
+            const int c0 = R.cell_base + lr * R.dim[0];
+            v = (start[c0 + R.dim[0]] - start[c0] + kChunk - 1) / kChunk;
+        }
+        nch[r] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static int blocks_for(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > 65535) b = 65535;
+    return (int)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_fill_u32(void* p, int64_t n, uint32_t v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(256), 0, st, (uint32_t*)p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_count_b(const double* xyz, int64_t n, const int32_t* off, int nreg, const RegionDesc* reg,
+                               double h, int32_t* cell_of, int32_t* count, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cell_count_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, n, off, nreg, reg, h, cell_of,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* buf, int64_t n, const int32_t* off,
+                            int nreg, const RegionDesc* reg, double h, const int32_t* cell_of, const int32_t* start,
+                            int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scatter_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, w, buf, n, off, nreg, reg, h, cell_of,
+                       start, fill, pw, pl, flags, orig);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_chunks_b(const int32_t* start, const RegionDesc* reg, const int32_t* row_base, int nreg,
+                               int nrow_tot, int32_t* nch, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_row_chunks_b, dim3(blocks_for(nrow_tot + 1)), dim3(256), 0, st, start, reg, row_base, nreg,
+                       nrow_tot, nch);
+    return hipGetLastError();
+}
+
 }  // namespace gs
 

@@ -33,6 +33,36 @@ struct GsExtra {
 };
 void extra_release(GsExtra& x);
 
+// Batched independent regions (batched.cpp): host description of the regions
+// and the device buffers / captured hipGraphs of the batched pipeline.
+struct GsBatch {
+    int nreg = 0;
+    bool valid = false;
+    std::vector<int64_t> off, ndata;
+    std::vector<double> bbox;            // [nreg][6]
+    std::vector<long double> wd, wr;     // per-region weight sums
+    double wmax = 1.0, xmax = 0.0;
+    DevBuf<int32_t> d_off, d_row_base, cell_of, count, cell_start, nch, row_chunk, scan_tmp, queue, orig;
+    DevBuf<unsigned char> d_regions, params;
+    DevBuf<double4> pw;
+    DevBuf<float4> pl;
+    DevBuf<uint32_t> flags;
+    DevBuf<unsigned long long> out;
+    struct Plan {
+        gs_params p{};
+        int m = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int nbt = 0;
+        int fx_shift = 0;
+        int mst = 0;
+        DevBuf<unsigned char> blob;      // regions, bins and stencil rows read by the captured kernels
+    };
+    std::vector<Plan> plans;
+    std::vector<std::pair<std::string, int>> tuned;   // parameter key -> tuned cells per rmax
+};
+void batch_release(GsBatch& b);
+
 struct CtxView {
     hipStream_t stream;
     hipEvent_t* ev;
@@ -47,12 +77,20 @@ struct CtxView {
     const uint32_t* flags;
     const int32_t* orig;
     const int32_t* cell_start;
+    const float4* pl;
+    const int32_t* row_chunk;
+    const double* w;            // raw weights, original order (NULL for unit weights)
     int32_t n_chunks;
     GsExtra* extra;
+    GsBatch* batch;
+    const double* xyz;           // catalogue, original order
+    const int32_t* buf;
 };
 
 CtxView view(gs_ctx* c);
 int ctx_set_device(gs_ctx* c);
+int ctx_upload(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf, int64_t n);
+double round11(double h);
 int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
 int check_params(const gs_params* p, int mode);
 int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B);

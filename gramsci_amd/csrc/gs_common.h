@@ -43,6 +43,19 @@ struct BinDesc {
     int32_t fx_shift2, fx_shift3;
 };
 
+// One independent region of a batched run (a domain-decomposed sub-volume
+// with its buffer shell, or a jackknife region): its own grid, its own points
+// (data first, then randoms) and its own output histogram.
+struct RegionDesc {
+    double lo[3];
+    int32_t dim[3];
+    int32_t cell_base;       // first global cell id of the region
+    int32_t row_base;        // first global x-row of the region
+    int32_t pt_base;         // first point (input order and cell-sorted order)
+    int32_t n_data;          // data points of the region (the rest are randoms)
+    int32_t pad;
+};
+
 // Half-stencil rows of the pair kernel: cells (ax+dx, ay+dy, az+dz) with dx in [x0, x1].
 struct Row {
     int8_t dy, dz, x0, x1;

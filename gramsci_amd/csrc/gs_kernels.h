@@ -16,6 +16,16 @@ hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf
                           double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
 hipError_t launch_row_chunks(const int32_t* start, int nx, int nrow, int32_t* nch, hipStream_t st);
 
+// batched regions (grid.hip)
+hipError_t launch_fill_u32(void* p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_cell_count_b(const double* xyz, int64_t n, const int32_t* off, int nreg, const RegionDesc* reg,
+                               double h, int32_t* cell_of, int32_t* count, hipStream_t st);
+hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* buf, int64_t n, const int32_t* off,
+                            int nreg, const RegionDesc* reg, double h, const int32_t* cell_of, const int32_t* start,
+                            int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
+hipError_t launch_row_chunks_b(const int32_t* start, const RegionDesc* reg, const int32_t* row_base, int nreg,
+                               int nrow_tot, int32_t* nch, hipStream_t st);
+
 // Arguments of the persistent 2PCF pair kernel (pair2.hip).
 struct PairArgs {
     const double4* pw;          // sorted points: x, y, z, raw weight
@@ -34,6 +44,13 @@ struct PairArgs {
     int32_t n_chunks;
     int32_t reps;               // LDS histogram replicas (power of two <= 32)
     int32_t mirror;             // (r, mu) with no buffer points: one update per unordered pair
+    // batched regions (nreg > 0): per-region grids; each wave keeps a private
+    // histogram tagged with its current region; n_chunks is read from n_chunks_dev
+    const RegionDesc* regions;
+    const int32_t* row_base;    // nreg+1 first global x-row per region
+    const int32_t* n_chunks_dev;
+    int32_t nreg;
+    int32_t pad2;
 };
 
 // Arguments of the persistent 3PCF triplet kernel (triplet3.hip).
@@ -87,11 +104,39 @@ size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist);
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);
 int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool weighted);
 
+// Arguments of the persistent 2PCF adjoint kernel (adjoint.hip).
+struct AdjArgs {
+    const double4* pw;
+    const float4* pl;
+    const uint32_t* flags;
+    const int32_t* orig;
+    const int32_t* cell_start;
+    const int32_t* row_chunk;
+    const Row* rows;            // full stencil
+    const BinDesc* bins;
+    const double* g2;           // dL/dN2 [nb][nm]
+    const double* g3;           // dL/dN3 [nb][nm]
+    int32_t* queue;
+    double* G;                  // dL/dwn per point, original order
+    unsigned long long* out_stats;
+    GridDesc g;
+    int32_t nrows;
+    int32_t n_chunks;
+    double inv_wd, inv_wr;
+};
+size_t adjoint2_lds_bytes(int nb, int nm, bool fp64);
+hipError_t launch_adjoint2(const AdjArgs& a, int nb, int nm, bool fp64, bool rsd, int nblocks, hipStream_t st);
+int adjoint2_occupancy(int nb, int nm, bool fp64, bool rsd);
+hipError_t launch_adj_partials(const double* G, const double* w, int64_t n, int64_t n_data, double inv_wd,
+                               double inv_wr, double* part, int nblk, hipStream_t st);
+hipError_t launch_adj_finish(const double* G, int64_t n, int64_t n_data, double s_d, double s_r, double inv_wd,
+                             double inv_wr, double* out, hipStream_t st);
+
 // Dynamic LDS bytes of the pair kernel for these parameters.
-size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted);
+size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched = false);
 hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, int nblocks,
                         hipStream_t st);
 // Work-groups per CU the pair kernel can keep resident for this LDS size.
-int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted);
+int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool batched = false);
 
 }  // namespace gs

@@ -39,78 +39,18 @@
 // bit 32 (no carries, so no returning atomics).  A wave that has issued 2^29
 // increments drains the histogram with atomic exchanges into the global
 // 128-bit accumulators; the rest is drained when the work-group ends.
+#include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
 namespace {
 
+using namespace dev;
+
 struct Stats {
     unsigned long long cand, rechk, over;
 };
-
-__device__ __forceinline__ void glb_add128(unsigned long long* slot, unsigned long long lo, unsigned long long hi)
-{
-    unsigned long long old = atomicAdd(slot, lo);
-    if (old + lo < old) hi += 1ull;
-    if (hi) atomicAdd(slot + 1, hi);
-}
-
-// Reference arithmetic in FP64, no contraction: s (kdtree2.F90:1689, gramsci.F90:1532).
-__device__ __forceinline__ double exact_s(double ax, double ay, double az, double bx, double by, double bz)
-{
-    double dx = __dsub_rn(bx, ax), dy = __dsub_rn(by, ay), dz = __dsub_rn(bz, az);
-    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-}
-
-// 0-based bin of s: #{k in 1..nb : s >= t[k]}; nb means "out of range" (reference bin nbins+1).
-__device__ __forceinline__ int exact_bin(double s, const double* t, int nb)
-{
-    int lo = 0, hi = nb;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (t[mid] <= s) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// mu bin from t = (theta+1)*mu_scale (gramsci.F90:326-328), 0-based.  The
-// reference stores floor(t)+1 in an int8 before clamping; that wrap is kept.
-__device__ __forceinline__ int mu_from_t(double t, int nm)
-{
-    int v = (int)floor(t) + 1;
-    v = (int)(int8_t)v;
-    v = v < 1 ? 1 : (v > nm ? nm : v);
-    return v - 1;
-}
-
-// mu bins of apex a -> neighbour b and of b -> a (gramsci.F90:320-325).
-__device__ __forceinline__ void exact_mu(double ax, double ay, double az, double bx, double by, double bz, double ms,
-                                         int nm, int& m, int& mr)
-{
-    double m1 = __dmul_rn(0.5, __dadd_rn(ax, bx));
-    double m2 = __dmul_rn(0.5, __dadd_rn(ay, by));
-    double m3 = __dmul_rn(0.5, __dadd_rn(az, bz));
-    double w1 = __dsub_rn(ax, bx), w2 = __dsub_rn(ay, by), w3 = __dsub_rn(az, bz);
-    double th = __dadd_rn(__dadd_rn(__dmul_rn(m1, w1), __dmul_rn(m2, w2)), __dmul_rn(m3, w3));
-    double mm = __dadd_rn(__dadd_rn(__dmul_rn(m1, m1), __dmul_rn(m2, m2)), __dmul_rn(m3, m3));
-    double ww = __dadd_rn(__dadd_rn(__dmul_rn(w1, w1), __dmul_rn(w2, w2)), __dmul_rn(w3, w3));
-    double den = __dmul_rn(__dsqrt_rn(mm), __dsqrt_rn(ww));
-    th = __ddiv_rn(th, den);
-    m = mu_from_t(__dmul_rn(__dadd_rn(th, 1.0), ms), nm);
-    mr = mu_from_t(__dmul_rn(__dadd_rn(-th, 1.0), ms), nm);
-}
-
-// floor(x) split at bit 32, x = w_i * (w_j * 2^shift) < 2^61 (the scale is
-// folded into the staged w_j; scaling by 2^shift is exact):
-// q = hi * 2^32 + lo, lo < 2^32, hi < 2^29.  Truncation moves each term by
-// less than 2^-shift, at most 2^-61 of the largest product.
-__device__ __forceinline__ void quantise(double x, uint32_t& lo, uint32_t& hi)
-{
-    const double h = floor(x * 0x1p-32);
-    hi = (uint32_t)h;
-    lo = (uint32_t)fma(-h, 0x1p32, x);
-}
 
 template <bool RSD, bool W>
 __device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw, int reps, int rep, int nb, int nm,
@@ -193,32 +133,15 @@ __device__ void drain(uint32_t* hc, unsigned long long* hw, int nbt, int nm, int
     }
 }
 
-__device__ __forceinline__ int find_row(const int32_t* row_chunk, int nrow, int chunk)
-{
-    // largest r with row_chunk[r] <= chunk
-    int lo = 0, hi = nrow - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (row_chunk[mid] <= chunk) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 }  // namespace
 
-size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted)
+size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched)
 {
     const size_t nbt = (size_t)3 * nb * nm;
+    const size_t copies = batched ? (size_t)(kWG / kWave) : 1;   // batched: one histogram per wave
     size_t b = 0;
-    b += ((nbt * reps * 4 + 15) / 16) * 16;                  // counts
-    if (weighted) b += nbt * 16;                              // split fixed-point sums
+    b += ((nbt * reps * 4 * copies + 15) / 16) * 16;          // counts
+    if (weighted) b += nbt * 16 * copies;                     // split fixed-point sums
     b += ((size_t)(kMaxBins + 1) * 8);                        // ok_lo/ok_hi as float2
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
     const int waves = kWG / kWave;
@@ -235,18 +158,23 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     const int nb = B.nb;
     const int nm = RSD ? B.nm : 1;
     const int nbt = 3 * nb * nm;
-    const int reps = a.reps;
+    const bool batched = a.nreg > 0;
+    const int reps = batched ? 1 : a.reps;
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wv = tid / kWave;
     const int rep = lane & (reps - 1);
     const bool mirror = RSD && a.mirror;
+    const int copies = batched ? kWG / kWave : 1;
 
     unsigned char* p = smem;
-    uint32_t* hc = (uint32_t*)p;
-    p += ((size_t)nbt * reps * 4 + 15) / 16 * 16;
-    unsigned long long* hw = (unsigned long long*)p;
-    if (W) p += (size_t)nbt * 16;
+    uint32_t* hc0 = (uint32_t*)p;
+    p += ((size_t)nbt * reps * 4 * copies + 15) / 16 * 16;
+    unsigned long long* hw0 = (unsigned long long*)p;
+    if (W) p += (size_t)nbt * 16 * copies;
+    // the histogram this wave adds to: shared by the work-group, or wave-private in batched mode
+    uint32_t* hc = batched ? hc0 + (size_t)wv * nbt : hc0;
+    unsigned long long* hw = batched ? hw0 + (size_t)wv * 2 * nbt : hw0;
     float2* okb = (float2*)p;
     p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
@@ -261,9 +189,9 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     double* tw = FP64 ? (tz + kWave) : (double*)(tl + kWave);
     uint32_t* tf = (uint32_t*)(tw + kWave);
 
-    for (int k = tid; k < nbt * reps; k += kWG) hc[k] = 0;
+    for (int k = tid; k < nbt * reps * copies; k += kWG) hc0[k] = 0;
     if (W)
-        for (int k = tid; k < 2 * nbt; k += kWG) hw[k] = 0;
+        for (int k = tid; k < 2 * nbt * copies; k += kWG) hw0[k] = 0;
     for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
     for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
     __syncthreads();
@@ -274,10 +202,10 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     const double ms = B.mu_scale;
     const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
     const int shift = B.fx_shift2;
-    const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
-    const int nrow = ny * nz;
+    const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
+    const int n_chunks = batched ? *a.n_chunks_dev : a.n_chunks;
     const float h32 = a.g.h32;
-    const double h = a.g.h;
+    int cur_r = -1;                 // region of this wave's private histogram (batched mode)
 
     Stats st{0, 0, 0};
     unsigned long long bound = 0;   // LDS increments issued by this wave since its last drain
@@ -286,23 +214,33 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
 
     while (true) {
         int item = -1;
-        if (lane == 0) {
-            while (tried < 8) {
-                const int sidx = (xcd + tried) & 7;
-                const int lo = (int)(((long long)a.n_chunks * sidx) >> 3);
-                const int hi = (int)(((long long)a.n_chunks * (sidx + 1)) >> 3);
-                const int k = atomicAdd(&a.queue[sidx], 1);
-                if (lo + k < hi) { item = lo + k; break; }
-                ++tried;
-            }
-        }
+        if (lane == 0) item = next_item(a.queue, n_chunks, xcd, tried);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item < 0) break;
 
-        const int row = find_row(a.row_chunk, nrow, item);
-        const int q = item - a.row_chunk[row];
+        int nx, ny, nz, cb, row;
+        const int grow = upper_index(a.row_chunk, batched ? a.row_base[a.nreg] : nrow_all, item);
+        const int q = item - a.row_chunk[grow];
+        if (batched) {
+            const int R = upper_index(a.row_base, a.nreg, grow);
+            const RegionDesc& rd = a.regions[R];
+            nx = rd.dim[0]; ny = rd.dim[1]; nz = rd.dim[2];
+            cb = rd.cell_base;
+            row = grow - rd.row_base;
+            if (R != cur_r) {
+                if (cur_r >= 0)
+                    drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + (size_t)cur_r * nbt,
+                          a.out_w + (size_t)cur_r * 2 * nbt);
+                cur_r = R;
+                bound = 0;
+            }
+        } else {
+            nx = a.g.dim[0]; ny = a.g.dim[1]; nz = a.g.dim[2];
+            cb = 0;
+            row = grow;
+        }
         const int cy = row % ny, cz = row / ny;
-        const int rs = a.cell_start[row * nx], re = a.cell_start[(row + 1) * nx];
+        const int rs = a.cell_start[cb + row * nx], re = a.cell_start[cb + (row + 1) * nx];
         const int i0 = rs + q * kChunk;
         const int n_i = min(kChunk, re - i0);
         const int i = i0 + lane;
@@ -328,7 +266,7 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
             const int bx0 = max(cx_first + rw.x0, 0), bx1 = min(cx_last + rw.x1, nx - 1);
             if (bx0 > bx1) continue;
             const bool self = (rw.dy == 0 && rw.dz == 0);
-            const int rb = (bz * ny + by) * nx;
+            const int rb = cb + (bz * ny + by) * nx;
             int js = a.cell_start[rb + bx0];
             const int je = a.cell_start[rb + bx1 + 1];
             if (self) js = max(js, i0 + 1);
@@ -422,14 +360,22 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                 // keep every u32 LDS counter (and the split sums) below 2^32: each wave drains
                 // after 2^29 of its own increments, so four waves stay below 2^31 in total
                 if (bound > (1ull << 29)) {
-                    drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt, a.out_w);
+                    const size_t ro = batched ? (size_t)cur_r : 0;
+                    drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + ro * nbt,
+                          a.out_w + ro * 2 * nbt);
                     bound = 0;
                 }
             }
         }
     }
-    __syncthreads();
-    drain(hc, hw, nbt, nm, reps, W, mirror, tid, kWG, a.out_cnt, a.out_w);
+    if (batched) {
+        if (cur_r >= 0)
+            drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + (size_t)cur_r * nbt,
+                  a.out_w + (size_t)cur_r * 2 * nbt);
+    } else {
+        __syncthreads();
+        drain(hc, hw, nbt, nm, reps, W, mirror, tid, kWG, a.out_cnt, a.out_w);
+    }
 
     // wave-reduce the statistics
     for (int off = 32; off > 0; off >>= 1) {
@@ -459,7 +405,7 @@ static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, hipStream
 hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, int nblocks,
                         hipStream_t st)
 {
-    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, a.reps, fp64, weighted);
+    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, a.reps, fp64, weighted, a.nreg > 0);
     if (fp64) {
         if (rsd) return weighted ? launch_t<true, true, true>(a, lds, nblocks, st) : launch_t<true, true, false>(a, lds, nblocks, st);
         return weighted ? launch_t<true, false, true>(a, lds, nblocks, st) : launch_t<true, false, false>(a, lds, nblocks, st);
@@ -476,9 +422,9 @@ static int occ_t(size_t lds)
     return nb > 0 ? nb : 1;
 }
 
-int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted)
+int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool batched)
 {
-    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted);
+    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted, batched);
     if (fp64) {
         if (rsd) return weighted ? occ_t<true, true, true>(lds) : occ_t<true, true, false>(lds);
         return weighted ? occ_t<true, false, true>(lds) : occ_t<true, false, false>(lds);

@@ -19,31 +19,12 @@
 // Counts go to an LDS histogram [cell][16 random patterns] (or straight to
 // global memory when it does not fit); weighted runs add FP64 sums of the
 // quad products per pattern and of the six disc pair products (:754-818).
+#include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
-namespace {
-
-__device__ __forceinline__ double exact_s(double ax, double ay, double az, double bx, double by, double bz)
-{
-    double dx = __dsub_rn(bx, ax), dy = __dsub_rn(by, ay), dz = __dsub_rn(bz, az);
-    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-}
-
-// exact 0-based bin, -1 if not kept, nb if the reference bin would overflow
-__device__ __forceinline__ int bin_of(double s, const double* t, int nb)
-{
-    if (!(s > t[0] && s < t[nb + 1])) return -1;
-    int lo = 0, hi = nb;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (t[mid] <= s) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-}  // namespace
+using namespace dev;
 
 size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist)
 {
@@ -100,15 +81,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
 
     while (true) {
         if (tid == 0) {
-            int it = -1;
-            while (tried < 8) {
-                const int sidx = (xcd + tried) & 7;
-                const int lo = (int)(((long long)a.n_items * sidx) >> 3);
-                const int hi = (int)(((long long)a.n_items * (sidx + 1)) >> 3);
-                const int k = atomicAdd(&a.queue[sidx], 1);
-                if (lo + k < hi) { it = lo + k; break; }
-                ++tried;
-            }
+            const int it = next_item(a.queue, a.n_items, xcd, tried);
             s_item = it;
             s_k = 0;
             s_over = 0;

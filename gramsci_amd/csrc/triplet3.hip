@@ -18,43 +18,12 @@
 //      evaluated directly from the neighbours' coordinates - in FP64, or in
 //      FP32 relative to the apex with the same guard band and FP64 recheck as
 //      the pair kernel - instead of a binary search in a neighbour list.
+#include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
-namespace {
-
-__device__ __forceinline__ void glb_add128(unsigned long long* slot, unsigned long long lo, unsigned long long hi)
-{
-    unsigned long long old = atomicAdd(slot, lo);
-    if (old + lo < old) hi += 1ull;
-    if (hi) atomicAdd(slot + 1, hi);
-}
-
-__device__ __forceinline__ double exact_s(double ax, double ay, double az, double bx, double by, double bz)
-{
-    double dx = __dsub_rn(bx, ax), dy = __dsub_rn(by, ay), dz = __dsub_rn(bz, az);
-    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
-}
-
-__device__ __forceinline__ int exact_bin(double s, const double* t, int nb)
-{
-    int lo = 0, hi = nb;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (t[mid] <= s) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ void quantise(double x, uint32_t& lo, uint32_t& hi)
-{
-    const double h = floor(x * 0x1p-32);
-    hi = (uint32_t)h;
-    lo = (uint32_t)fma(-h, 0x1p32, x);
-}
-
-}  // namespace
+using namespace dev;
 
 size_t triplet3_lds_bytes(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted)
 {
@@ -128,15 +97,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 
     while (true) {
         if (tid == 0) {
-            int it = -1;
-            while (tried < 8) {
-                const int sidx = (xcd + tried) & 7;
-                const int lo = (int)(((long long)a.n_items * sidx) >> 3);
-                const int hi = (int)(((long long)a.n_items * (sidx + 1)) >> 3);
-                const int k = atomicAdd(&a.queue[sidx], 1);
-                if (lo + k < hi) { it = lo + k; break; }
-                ++tried;
-            }
+            const int it = next_item(a.queue, a.n_items, xcd, tried);
             s_item = it;
             s_k = 0;
             s_over = 0;

@@ -94,6 +94,31 @@ class Engine:
                                                nd, n - nd))
         self.n_data, self.n_rand = nd, n - nd
 
+    def set_catalogue_batched(self, xyz, w, buf, off, ndata):
+        """Independent regions: region r is points off[r]:off[r+1] (data first,
+        ndata[r] of them, then randoms); buf flags never-apex shell points."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        w = None if w is None else np.ascontiguousarray(w, dtype=np.float64)
+        buf = None if buf is None else np.ascontiguousarray(buf, dtype=np.int32)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        ndata = np.ascontiguousarray(ndata, dtype=np.int64)
+        L.check(self._lib.gs_set_catalogue_batched(self._ctx, L.ptr(xyz), L.ptr(w), L.ptr(buf, ctypes.c_int32),
+                                                   L.ptr(off, ctypes.c_int64), L.ptr(ndata, ctypes.c_int64),
+                                                   len(ndata)))
+        self.nreg = len(ndata)
+
+    def pcf2_batched(self, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
+        p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
+        R = self.nreg
+        cnt = np.zeros((R, 3, nbins, nmu), np.uint64)
+        wsum = np.zeros((R, 3, nbins, nmu))
+        n2 = np.zeros((R, nbins, nmu))
+        n3 = np.zeros((R, nbins, nmu))
+        st = L.gs_stats()
+        L.check(self._lib.gs_count_2pcf_batched(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64),
+                                                L.ptr(wsum), L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
+        return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, xi=_xi(n2, n3), stats=st.as_dict())
+
     # ------------------------------------------------------------ counts
     @staticmethod
     def params(rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
@@ -109,6 +134,28 @@ class Engine:
         L.check(self._lib.gs_count_2pcf(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(wsum),
                                         L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
         return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, xi=_xi(n2, n3), stats=st.as_dict())
+
+    def adjoint2(self, dl_dn2, dl_dn3, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0,
+                 out=None):
+        """dL/dw for the raw weights of the current catalogue, given dL/dN2 and
+        dL/dN3 of shape (nbins, nmu).  Returns a numpy array, or fills `out`
+        (a float64 torch CUDA tensor of n elements) on the device."""
+        p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
+        g2 = np.ascontiguousarray(dl_dn2, dtype=np.float64).reshape(nbins, nmu)
+        g3 = np.ascontiguousarray(dl_dn3, dtype=np.float64).reshape(nbins, nmu)
+        st = L.gs_stats()
+        if out is not None:
+            import torch
+            if out.dtype != torch.float64 or not out.is_cuda or out.numel() != self.n_data + self.n_rand:
+                raise ValueError("out must be a float64 CUDA tensor with one entry per point")
+            L.check(self._lib.gs_adjoint_2pcf_device(self._ctx, ctypes.byref(p), L.ptr(g2), L.ptr(g3),
+                                                     ctypes.c_void_p(out.data_ptr()), ctypes.byref(st)))
+            return out
+        res = np.zeros(self.n_data + self.n_rand)
+        L.check(self._lib.gs_adjoint_2pcf(self._ctx, ctypes.byref(p), L.ptr(g2), L.ptr(g3), L.ptr(res),
+                                          ctypes.byref(st)))
+        self.last_stats = st.as_dict()
+        return res
 
     def pcf3(self, rmin, rmax, nbins, log_bins=False, precision="fp32", cell_div=0):
         p = self.params(rmin, rmax, nbins, 1, log_bins, precision, cell_div)

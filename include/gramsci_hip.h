@@ -120,6 +120,34 @@ int gs_count_3pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
 int gs_count_4pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* n2, double* n3,
                   double* disc_n, double* disc_r, gs_stats* st);
 
+/* Batched independent regions (domain-decomposed sub-volumes with their
+ * buffer shells, jackknife regions): region r holds points [off[r], off[r+1])
+ * of xyz/w/buf, its first ndata[r] points are data and the rest randoms; buf
+ * flags the shell points that are neighbours but never apexes.  This is the
+ * reference's split-run-resum workflow (domain_decomposition.f90:49-74, -cut
+ * mode gramsci.F90:1224-1231, buffer skip :363) with every region in ONE
+ * persistent launch, captured in a hipGraph and replayed.  Each region is
+ * normalised on its own, exactly as a separate -gal/-ran run would be.
+ * Host pointers; off[nreg+1], ndata[nreg]. */
+int gs_set_catalogue_batched(gs_ctx* ctx, const double* xyz, const double* w, const int32_t* buf,
+                             const int64_t* off, const int64_t* ndata, int nreg);
+/* Per-region 2PCF of the batched catalogue: cnt/wsum [nreg][3][nbins][nmu],
+ * n2/n3 [nreg][nbins][nmu].  p->cell_div = 0 selects the cell size by timing
+ * the captured pipeline for 2..6 cells per rmax (cached per parameter set). */
+int gs_count_2pcf_batched(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
+                          double* n2, double* n3, gs_stats* st);
+
+/* Adjoint of the weighted 2PCF: given dl_dn2 = dL/dN2 and dl_dn3 = dL/dN3
+ * [nbins][nmu] (host memory) for the catalogue of the context, writes
+ * dL/dw_k for the RAW weights w_k (the normalisation of gramsci.F90:1390-1391
+ * is differentiated through).  The pairs, bins and buffer rule are those of
+ * gs_count_2pcf (query_graph_2pcf, gramsci.F90:352-380); the reference has no
+ * derivative.  dl_dw[n] in host memory, or d_dl_dw[n] in device memory. */
+int gs_adjoint_2pcf(gs_ctx* ctx, const gs_params* p, const double* dl_dn2, const double* dl_dn3,
+                    double* dl_dw, gs_stats* st);
+int gs_adjoint_2pcf_device(gs_ctx* ctx, const gs_params* p, const double* dl_dn2, const double* dl_dn3,
+                           double* d_dl_dw, gs_stats* st);
+
 /* One-shot forms used by the Fortran host: create a context on device 0 (kept
  * for the process), upload, count.  Same outputs as above. */
 int gs_pcf2(const double* xyz, const double* w, const int32_t* buf, int64_t n_data, int64_t n_rand,

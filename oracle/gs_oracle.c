@@ -509,3 +509,52 @@ out:
     free(tab); free(wn); free(acc); free(c);
     return rc;
 }
+
+/* -------------------------------------------------------------- adjoint -- */
+
+int or_adjoint_2pcf(const double* xyz, const double* w, const int32_t* buf,
+                    int64_t n_data, int64_t n_rand, const or_params* p,
+                    const double* gN2, const double* gN3, double* dldw)
+{
+    /* N2 = sum over kept ordered pairs (i apex) of wn_i wn_j, N3 the same over RR
+     * pairs (gramsci.F90:373-376).  dL/dwn_k collects every ordered pair that
+     * contains k, as apex or as neighbour; the raw-weight gradient follows from
+     * wn = w/Wd (data) and wn = -w/Wr (randoms). */
+    int rc = check_params(p);
+    if (rc) return rc;
+    const int64_t n = n_data + n_rand;
+    const int nm = p->nmu;
+    double* wn = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    long double* G = (long double*)calloc((size_t)(n > 0 ? n : 1), sizeof(long double));
+    if (!wn || !G) { free(wn); free(G); return OR_ENOMEM; }
+    or_normalise(w, n_data, n_rand, wn);
+    grid_t g;
+    if ((rc = grid_build(&g, xyz, n, grid_h(xyz, n, p->rmax)))) { free(wn); free(G); grid_free(&g); return rc; }
+    for (int64_t i = 0; i < n; ++i) {
+        if (buf && buf[i] == 1) continue;
+        const int ri = i >= n_data;
+        GRID_FOR_NEIGHBOURS(&g, xyz, i, j, {
+            int b = or_pair_rbin(p, &xyz[3 * i], &xyz[3 * j]);
+            if (b == 0 || b > p->nbins) continue;
+            int m = or_pair_mubin(p, &xyz[3 * i], &xyz[3 * j]);
+            const int rj = j >= n_data;
+            const int64_t o = (int64_t)(b - 1) * nm + (m - 1);
+            long double c = gN2[o];
+            if (ri && rj) c += gN3[o];
+            /* d(wn_i wn_j)/dwn_i = wn_j, d/dwn_j = wn_i */
+            G[i] += c * wn[j];
+            G[j] += c * wn[i];
+        });
+    }
+    long double sd = 0, sr = 0, Wd = 0, Wr = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const double wk = w ? w[k] : 1.0;
+        if (k < n_data) { sd += (long double)wn[k] * G[k]; Wd += wk; }
+        else { sr += (long double)wn[k] * G[k]; Wr += wk; }
+    }
+    for (int64_t k = 0; k < n; ++k)
+        dldw[k] = k < n_data ? (double)((G[k] - sd) / Wd) : (double)(-(G[k] + sr) / Wr);
+    grid_free(&g);
+    free(wn); free(G);
+    return OR_OK;
+}

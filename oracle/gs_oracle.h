@@ -82,6 +82,14 @@ int or_4pcf(const double* xyz, const double* w, const int32_t* buf,
             uint64_t* cnt, double* n2, double* n3, double* disc_n, double* disc_r,
             uint64_t* n_quads);
 
+/* Gradient of a loss L(N2, N3) of the 2PCF with respect to the RAW weights
+ * (before the normalisation of gramsci.F90:1390-1391), given gN2 = dL/dN2 and
+ * gN3 = dL/dN3 [nbins][nmu].  Brute force over the ordered pairs, long double
+ * accumulation.  dldw[n]. */
+int or_adjoint_2pcf(const double* xyz, const double* w, const int32_t* buf,
+                    int64_t n_data, int64_t n_rand, const or_params* p,
+                    const double* gN2, const double* gN3, double* dldw);
+
 #ifdef __cplusplus
 }
 #endif

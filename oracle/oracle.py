@@ -56,7 +56,8 @@ def lib():
         L.or_2pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, u64p]
         L.or_3pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, u64p]
         L.or_4pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, dp, u64p]
-        for f in ("or_2pcf", "or_3pcf", "or_4pcf", "or_pair_rbin", "or_pair_mubin",
+        L.or_adjoint_2pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, dp, dp, dp]
+        for f in ("or_2pcf", "or_3pcf", "or_4pcf", "or_adjoint_2pcf", "or_pair_rbin", "or_pair_mubin",
                   "or_n3cfg", "or_n4cfg", "or_bintable3", "or_bintable4"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
@@ -161,6 +162,21 @@ def pcf4(xyz, w, n_data, p: Params, buf=None):
     if rc:
         raise ValueError(f"or_4pcf failed: {rc}")
     return dict(cnt=cnt, n2=n2, n3=n3, disc_n=dn, disc_r=dr, n_quads=int(q.value))
+
+
+def adjoint2(xyz, w, n_data, p: Params, gN2, gN3, buf=None):
+    """dL/dw (raw weights) for dL/dN2 = gN2, dL/dN3 = gN3 of shape (nbins, nmu)."""
+    xyz, w, buf = _prep(xyz, w, buf)
+    n = xyz.shape[0]
+    g2 = np.ascontiguousarray(gN2, dtype=np.float64).reshape(p.nbins, p.nmu)
+    g3 = np.ascontiguousarray(gN3, dtype=np.float64).reshape(p.nbins, p.nmu)
+    out = np.zeros(n)
+    rc = lib().or_adjoint_2pcf(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
+                               n_data, n - n_data, ctypes.byref(p.c()), _ptr(g2, ctypes.c_double),
+                               _ptr(g3, ctypes.c_double), _ptr(out, ctypes.c_double))
+    if rc:
+        raise ValueError(f"or_adjoint_2pcf failed: {rc}")
+    return out
 
 
 # ------------------------------------------------------------------ reference binary

@@ -89,6 +89,7 @@ int or_pair_mubin(const or_params* p, const double* ci, const double* nj)
     double theta = m1 * w1 + m2 * w2 + m3 * w3;
     theta = theta / (sqrt(m1 * m1 + m2 * m2 + m3 * m3) * sqrt(w1 * w1 + w2 * w2 + w3 * w3));
     int mu = (int)floor((theta + 1.0) * mu_scale) + 1;
+    mu = (int)(int8_t)mu;                          /* mu1 is integer(int8) (:25) */
     if (mu < 1) mu = 1;
     if (mu > p->nmu) mu = p->nmu;
     return mu;
@@ -281,11 +282,13 @@ int or_bintable4(int nb, int a, int b, int c, int d)
 
 /* --------------------------------------------------------------- graph -- */
 /* The reference's neighbour graph (gramsci.F90:288-348): for each point the
- * kept neighbours, sorted by id (sorting_module.F90:36-47), with their bin. */
+ * kept neighbours, sorted by id (sorting_module.F90:36-47), with their bin
+ * and their mu bin (output(i)%mu, 1 when nmu < 2). */
 typedef struct {
     int64_t* off;   /* n+1 */
     int64_t* id;
     int8_t*  bin;
+    int8_t*  mu;
 } graph_t;
 
 static int cmp_pair(const void* a, const void* b)
@@ -314,18 +317,24 @@ static int graph_build(graph_t* G, const double* xyz, int64_t n, const or_params
     int64_t* tmp = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)(tot > 0 ? tot : 1));
     G->id = (int64_t*)malloc(sizeof(int64_t) * (size_t)(tot > 0 ? tot : 1));
     G->bin = (int8_t*)malloc((size_t)(tot > 0 ? tot : 1));
-    if (!tmp || !G->id || !G->bin) { free(tmp); grid_free(&g); return OR_ENOMEM; }
+    G->mu = (int8_t*)malloc((size_t)(tot > 0 ? tot : 1));
+    if (!tmp || !G->id || !G->bin || !G->mu) { free(tmp); grid_free(&g); return OR_ENOMEM; }
     for (int64_t i = 0; i < n; ++i) {
         int64_t k = 0;
         int64_t* t = tmp + 2 * G->off[i];
         GRID_FOR_NEIGHBOURS(&g, xyz, i, j, {
             int b = or_pair_rbin(p, &xyz[3 * i], &xyz[3 * j]);
-            if (b >= 1 && b <= p->nbins) { t[2 * k] = j; t[2 * k + 1] = b; ++k; }
+            if (b >= 1 && b <= p->nbins) {
+                t[2 * k] = j;
+                t[2 * k + 1] = b | ((int64_t)or_pair_mubin(p, &xyz[3 * i], &xyz[3 * j]) << 8);
+                ++k;
+            }
         });
         qsort(t, (size_t)k, 2 * sizeof(int64_t), cmp_pair);
         for (int64_t q = 0; q < k; ++q) {
             G->id[G->off[i] + q] = t[2 * q];
-            G->bin[G->off[i] + q] = (int8_t)t[2 * q + 1];
+            G->bin[G->off[i] + q] = (int8_t)(t[2 * q + 1] & 255);
+            G->mu[G->off[i] + q] = (int8_t)(t[2 * q + 1] >> 8);
         }
     }
     free(tmp);
@@ -333,7 +342,7 @@ static int graph_build(graph_t* G, const double* xyz, int64_t n, const or_params
     return OR_OK;
 }
 
-static void graph_free(graph_t* G) { free(G->off); free(G->id); free(G->bin); }
+static void graph_free(graph_t* G) { free(G->off); free(G->id); free(G->bin); free(G->mu); }
 
 /* find_dist (gramsci.F90:1579-1609): bin of id2 in id1's list, 0 if absent */
 static inline int find_dist(const graph_t* G, int64_t id1, int64_t id2)
@@ -349,6 +358,33 @@ static inline int find_dist(const graph_t* G, int64_t id1, int64_t id2)
 
 /* ---------------------------------------------------------------- 3PCF -- */
 
+/* find_normal (gramsci.F90:1611-1625) in its own precision: mu11, mu22 are
+ * default reals (the subtraction mu1-0.5 is single precision, the division by
+ * the double mu_scale and the "- 1.0" double, rounded to single on
+ * assignment); the rest of the expression is single precision, evaluated left
+ * to right; the result is stored in an integer(int8) before the clamp.  A
+ * negative radicand gives NaN, whose floor is not representable: any integer
+ * conversion of it ends in bin 1 after the int8 store and the clamp. */
+int or_normal_bin(int nmu, int mu1, int mu2)
+{
+    const double mu_scale = (double)(0.5f * (float)nmu);
+    const float a = (float)((double)((float)mu1 - 0.5f) / mu_scale - 1.0);
+    const float b = (float)((double)((float)mu2 - 0.5f) / mu_scale - 1.0);
+    volatile float x = 0.75f - a * a;
+    x = x - b * b;
+    x = x + a * b;
+    if (!(x >= 0.0f)) return 1;
+    volatile float y = 1.1547f * sqrtf(x);
+    y = y * (float)nmu;
+    int v = (int)floorf(y) + 1;
+    v = (int)(int8_t)v;
+    if (v < 1) v = 1;
+    if (v > nmu) v = nmu;
+    return v;
+}
+
+/* 3PCF, all configurations (gramsci.F90:463-545); with nmu >= 2 (RSD, :1219)
+ * the triplet goes to mu bin find_normal(mu(k1), mu(k2)) (:495-500). */
 int or_3pcf(const double* xyz, const double* w, const int32_t* buf,
             int64_t n_data, int64_t n_rand, const or_params* p,
             uint64_t* cnt, double* n2, double* n3, uint64_t* n_cand)
@@ -356,17 +392,21 @@ int or_3pcf(const double* xyz, const double* w, const int32_t* buf,
     int rc = check_params(p);
     if (rc) return rc;
     const int64_t n = n_data + n_rand;
-    const int nb = p->nbins, ncfg = or_n3cfg(nb);
+    const int nb = p->nbins, ncfg = or_n3cfg(nb), nm = p->nmu;
+    const int64_t nh = (int64_t)ncfg * nm;
     int* tab = (int*)malloc(sizeof(int) * (size_t)(nb * nb * nb));
+    int* ntab = (int*)malloc(sizeof(int) * (size_t)(nm * nm));
     double* wn = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
-    long double* acc = (long double*)calloc((size_t)ncfg * 2, sizeof(long double));
-    uint64_t* c = (uint64_t*)calloc((size_t)ncfg * 4, sizeof(uint64_t));
+    long double* acc = (long double*)calloc((size_t)nh * 2, sizeof(long double));
+    uint64_t* c = (uint64_t*)calloc((size_t)nh * 4, sizeof(uint64_t));
     graph_t G;
     memset(&G, 0, sizeof(G));
-    if (!tab || !wn || !acc || !c) { rc = OR_ENOMEM; goto out; }
+    if (!tab || !ntab || !wn || !acc || !c) { rc = OR_ENOMEM; goto out; }
     for (int a = 1; a <= nb; ++a)
         for (int b = 1; b <= nb; ++b)
             for (int d = 1; d <= nb; ++d) tab[((a - 1) * nb + (b - 1)) * nb + (d - 1)] = or_bintable3(nb, a, b, d);
+    for (int a = 1; a <= nm; ++a)
+        for (int b = 1; b <= nm; ++b) ntab[(a - 1) * nm + (b - 1)] = nm >= 2 ? or_normal_bin(nm, a, b) : 1;
     or_normalise(w, n_data, n_rand, wn);
     if ((rc = graph_build(&G, xyz, n, p))) goto out;
     uint64_t cand = 0;
@@ -385,23 +425,80 @@ int or_3pcf(const double* xyz, const double* w, const int32_t* buf,
                 const int ind3 = find_dist(&G, id1, id2);
                 if (ind3 == 0) continue;
                 const int cfg = tab[((ind1 - 1) * nb + (ind2 - 1)) * nb + (ind3 - 1)];
+                const int mu = ntab[(G.mu[o + k1] - 1) * nm + (G.mu[o + k2] - 1)];
+                const int64_t h = (int64_t)(cfg - 1) * nm + (mu - 1);
                 const int nr = ri + (id1 >= n_data) + (id2 >= n_data);
-                c[(cfg - 1) * 4 + nr]++;
+                c[h * 4 + nr]++;
                 const double prod = wi_w1 * wn[id2];
-                acc[cfg - 1] += (long double)prod;                       /* N2 (:507) */
-                if (nr == 3) acc[ncfg + cfg - 1] -= (long double)prod;   /* N3 (:505) */
+                acc[h] += (long double)prod;                    /* N2 (:501, :507) */
+                if (nr == 3) acc[nh + h] -= (long double)prod;  /* N3 (:498, :505) */
             }
         }
     }
-    for (int q = 0; q < ncfg; ++q) {
+    for (int64_t q = 0; q < nh; ++q) {
         if (n2) n2[q] = (double)acc[q];
-        if (n3) n3[q] = (double)acc[ncfg + q];
+        if (n3) n3[q] = (double)acc[nh + q];
     }
-    if (cnt) memcpy(cnt, c, sizeof(uint64_t) * (size_t)ncfg * 4);
+    if (cnt) memcpy(cnt, c, sizeof(uint64_t) * (size_t)nh * 4);
     if (n_cand) *n_cand = cand;
 out:
     graph_free(&G);
-    free(tab); free(wn); free(acc); free(c);
+    free(tab); free(ntab); free(wn); free(acc); free(c);
+    return rc;
+}
+
+/* Equilateral 3PCF (gramsci.F90:402-461, -equi): triangles whose three sides
+ * share one bin.  With nmu >= 2 only N2 is accumulated, at mu bin
+ * find_normal(mu(k1), mu(k2)) (:429-431); N3 stays 0. */
+int or_3pcf_equi(const double* xyz, const double* w, const int32_t* buf,
+                 int64_t n_data, int64_t n_rand, const or_params* p,
+                 uint64_t* cnt, double* n2, double* n3)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    const int64_t n = n_data + n_rand;
+    const int nb = p->nbins, nm = p->nmu;
+    const int64_t nh = (int64_t)nb * nm;
+    int* ntab = (int*)malloc(sizeof(int) * (size_t)(nm * nm));
+    double* wn = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    long double* acc = (long double*)calloc((size_t)nh * 2, sizeof(long double));
+    uint64_t* c = (uint64_t*)calloc((size_t)nh * 4, sizeof(uint64_t));
+    graph_t G;
+    memset(&G, 0, sizeof(G));
+    if (!ntab || !wn || !acc || !c) { rc = OR_ENOMEM; goto out; }
+    for (int a = 1; a <= nm; ++a)
+        for (int b = 1; b <= nm; ++b) ntab[(a - 1) * nm + (b - 1)] = nm >= 2 ? or_normal_bin(nm, a, b) : 1;
+    or_normalise(w, n_data, n_rand, wn);
+    if ((rc = graph_build(&G, xyz, n, p))) goto out;
+    for (int64_t i = 0; i < n; ++i) {
+        if (buf && buf[i] == 1) continue;                       /* :409 */
+        const int64_t o = G.off[i], k = G.off[i + 1] - o;
+        const int ri = i >= n_data;
+        for (int64_t k1 = 0; k1 < k; ++k1) {
+            const int ind1 = G.bin[o + k1];
+            const int64_t id1 = G.id[o + k1];
+            for (int64_t k2 = k1 + 1; k2 < k; ++k2) {           /* :419-420 */
+                if (G.bin[o + k2] != ind1) continue;            /* :422 */
+                const int64_t id2 = G.id[o + k2];
+                if (find_dist(&G, id1, id2) != ind1) continue;  /* :425-426 */
+                const int mu = ntab[(G.mu[o + k1] - 1) * nm + (G.mu[o + k2] - 1)];
+                const int64_t h = (int64_t)(ind1 - 1) * nm + (mu - 1);
+                const int nr = ri + (id1 >= n_data) + (id2 >= n_data);
+                c[h * 4 + nr]++;
+                const double prod = wn[i] * wn[id1] * wn[id2];
+                acc[h] += (long double)prod;                                /* :431, :437 */
+                if (nm < 2 && nr == 3) acc[nh + h] -= (long double)prod;   /* :434-435 */
+            }
+        }
+    }
+    for (int64_t q = 0; q < nh; ++q) {
+        if (n2) n2[q] = (double)acc[q];
+        if (n3) n3[q] = (double)acc[nh + q];
+    }
+    if (cnt) memcpy(cnt, c, sizeof(uint64_t) * (size_t)nh * 4);
+out:
+    graph_free(&G);
+    free(ntab); free(wn); free(acc); free(c);
     return rc;
 }
 
@@ -507,6 +604,104 @@ int or_4pcf(const double* xyz, const double* w, const int32_t* buf,
 out:
     graph_free(&G);
     free(tab); free(wn); free(acc); free(c);
+    return rc;
+}
+
+/* Regular-tetrahedron 4PCF (gramsci.F90:548-660, -tetra): apex i and
+ * neighbours k1 < k2 < k3 (id order) whose six sides all share one bin b.
+ * No minimum neighbour count (unlike query_4pcf_all2).  Per bin:
+ *   cnt[b][16]  counts by random pattern (bit0 apex, bit1 k1, bit2 k2, bit3 k3)
+ *   n2[b], n3[b]      N2(b,1,3) and N3(b,1,3) (N3: all four random, positive)
+ *   disc_n, disc_r [6][b]  discN(b,1,1,t), discR(b,1,1,t): triple products
+ *                           (:590-631). */
+int or_4pcf_tetra(const double* xyz, const double* w, const int32_t* buf,
+                  int64_t n_data, int64_t n_rand, const or_params* p,
+                  uint64_t* cnt, double* n2, double* n3, double* disc_n, double* disc_r)
+{
+    int rc = check_params(p);
+    if (rc) return rc;
+    const int64_t n = n_data + n_rand;
+    const int nb = p->nbins;
+    double* wn = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    long double* acc = (long double*)calloc((size_t)nb * 14, sizeof(long double));
+    uint64_t* c = (uint64_t*)calloc((size_t)nb * 16, sizeof(uint64_t));
+    graph_t G;
+    memset(&G, 0, sizeof(G));
+    if (!wn || !acc || !c) { rc = OR_ENOMEM; goto out; }
+    or_normalise(w, n_data, n_rand, wn);
+    if ((rc = graph_build(&G, xyz, n, p))) goto out;
+    long double* N2 = acc;
+    long double* N3 = acc + nb;
+    long double* DN = acc + 2 * nb;     /* [6][nb] */
+    long double* DR = acc + 8 * nb;     /* [6][nb] */
+    for (int64_t i = 0; i < n; ++i) {
+        if (buf && buf[i] == 1) continue;
+        const int64_t o = G.off[i], k = G.off[i + 1] - o;
+        const int ri = i >= n_data;
+        const double wi = wn[i];
+        for (int64_t k1 = 0; k1 < k; ++k1) {
+            const int b = G.bin[o + k1];
+            const int64_t id1 = G.id[o + k1];
+            const int r1 = id1 >= n_data;
+            const double w1 = wn[id1];
+            for (int64_t k2 = k1 + 1; k2 < k; ++k2) {
+                if (G.bin[o + k2] != b) continue;                       /* :564 */
+                const int64_t id2 = G.id[o + k2];
+                if (find_dist(&G, id1, id2) != b) continue;             /* :566-567 */
+                const int r2 = id2 >= n_data;
+                const double w2 = wn[id2];
+                for (int64_t k3 = k2 + 1; k3 < k; ++k3) {
+                    if (G.bin[o + k3] != b) continue;                   /* :571 */
+                    const int64_t id3 = G.id[o + k3];
+                    if (find_dist(&G, id2, id3) != b) continue;         /* :575-576 */
+                    if (find_dist(&G, id1, id3) != b) continue;         /* :578-579 */
+                    const int r3 = id3 >= n_data;
+                    const double w3 = wn[id3];
+                    const int q = b - 1;
+                    c[q * 16 + (ri | r1 << 1 | r2 << 2 | r3 << 3)]++;
+                    if (ri) {
+                        DN[5 * nb + q] += (long double)(wi * w3 * w1);
+                        if (r3 && r1) DR[5 * nb + q] += (long double)(wi * w3 * w1);
+                    }
+                    if (r3) {
+                        DN[0 * nb + q] += (long double)(w3 * wi * w1);
+                        if (ri && r1) DR[0 * nb + q] += (long double)(w3 * wi * w1);
+                    }
+                    if (ri) {
+                        DN[3 * nb + q] += (long double)(wi * w1 * w2);
+                        if (r1 && r2) DR[3 * nb + q] += (long double)(wi * w1 * w2);
+                    }
+                    if (r3) {
+                        DN[1 * nb + q] += (long double)(w3 * wi * w2);
+                        if (ri && r2) DR[1 * nb + q] += (long double)(w3 * wi * w2);
+                    }
+                    if (r1) {
+                        DN[2 * nb + q] += (long double)(w1 * wi * w3);
+                        if (ri && r3) DR[2 * nb + q] += (long double)(w1 * wi * w3);
+                    }
+                    if (ri) {
+                        DN[4 * nb + q] += (long double)(wi * w3 * w2);
+                        if (r3 && r2) DR[4 * nb + q] += (long double)(wi * w3 * w2);
+                    }
+                    const double prod = wi * w1 * w2 * w3;               /* :633 */
+                    N2[q] += (long double)prod;
+                    if (ri && r1 && r2 && r3) N3[q] += (long double)prod; /* :634-636 */
+                }
+            }
+        }
+    }
+    for (int q = 0; q < nb; ++q) {
+        if (n2) n2[q] = (double)N2[q];
+        if (n3) n3[q] = (double)N3[q];
+    }
+    for (int q = 0; q < 6 * nb; ++q) {
+        if (disc_n) disc_n[q] = (double)DN[q];
+        if (disc_r) disc_r[q] = (double)DR[q];
+    }
+    if (cnt) memcpy(cnt, c, sizeof(uint64_t) * (size_t)nb * 16);
+out:
+    graph_free(&G);
+    free(wn); free(acc); free(c);
     return rc;
 }
 

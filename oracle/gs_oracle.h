@@ -64,13 +64,24 @@ int or_n4cfg(int nbins);
 int or_bintable3(int nbins, int a, int b, int c);
 int or_bintable4(int nbins, int a, int b, int c, int d);
 
-/* 3PCF, isotropic (gramsci.F90:463-545 with find_dist :1579-1609).
- *   cnt[ncfg][4]  apex-ordered triplet counts by number of randoms among (i,k1,k2)
- *   n2, n3 [ncfg] the reference's N2(:,1,3), N3(:,1,3)
- *   n_cand        apex-pair candidates, sum_i C(k_i,2) over non-buffer apexes */
+/* find_normal (gramsci.F90:1611-1625): mu bin of a triplet from the mu bins
+ * mu1, mu2 (1-based) of its two apex sides; 1-based. */
+int or_normal_bin(int nmu, int mu1, int mu2);
+
+/* 3PCF, all configurations (gramsci.F90:463-545 with find_dist :1579-1609);
+ * nmu >= 2 is the reference's RSD 3PCF (find_normal, :495-500).
+ *   cnt[ncfg][nmu][4]  apex-ordered triplet counts by number of randoms among (i,k1,k2)
+ *   n2, n3 [ncfg][nmu] the reference's N2(:,:,3), N3(:,:,3)
+ *   n_cand             apex-pair candidates, sum_i C(k_i,2) over non-buffer apexes */
 int or_3pcf(const double* xyz, const double* w, const int32_t* buf,
             int64_t n_data, int64_t n_rand, const or_params* p,
             uint64_t* cnt, double* n2, double* n3, uint64_t* n_cand);
+
+/* Equilateral 3PCF (gramsci.F90:402-461).  cnt[nbins][nmu][4], n2, n3 [nbins][nmu]
+ * (N3 is only accumulated when nmu == 1, as in the reference). */
+int or_3pcf_equi(const double* xyz, const double* w, const int32_t* buf,
+                 int64_t n_data, int64_t n_rand, const or_params* p,
+                 uint64_t* cnt, double* n2, double* n3);
 
 /* 4PCF (gramsci.F90:662-861).  cells are (cfg, ind2, ind6): ncfg4*nbins*nbins.
  *   cnt[cell][16]     quad counts by random pattern (bit0 apex, bit1 k1, bit2 k2, bit3 k3)
@@ -81,6 +92,12 @@ int or_4pcf(const double* xyz, const double* w, const int32_t* buf,
             int64_t n_data, int64_t n_rand, const or_params* p,
             uint64_t* cnt, double* n2, double* n3, double* disc_n, double* disc_r,
             uint64_t* n_quads);
+
+/* Regular-tetrahedron 4PCF (gramsci.F90:548-660).  cnt[nbins][16],
+ * n2, n3 [nbins], disc_n, disc_r [6][nbins] (discN(b,1,1,t), discR(b,1,1,t)). */
+int or_4pcf_tetra(const double* xyz, const double* w, const int32_t* buf,
+                  int64_t n_data, int64_t n_rand, const or_params* p,
+                  uint64_t* cnt, double* n2, double* n3, double* disc_n, double* disc_r);
 
 /* Gradient of a loss L(N2, N3) of the 2PCF with respect to the RAW weights
  * (before the normalisation of gramsci.F90:1390-1391), given gN2 = dL/dN2 and

@@ -56,6 +56,9 @@ def lib():
         L.or_2pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, u64p]
         L.or_3pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, u64p]
         L.or_4pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, dp, u64p]
+        L.or_normal_bin.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_3pcf_equi.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp]
+        L.or_4pcf_tetra.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, dp]
         L.or_adjoint_2pcf.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, dp, dp, dp]
         for f in ("or_2pcf", "or_3pcf", "or_4pcf", "or_adjoint_2pcf", "or_pair_rbin", "or_pair_mubin",
                   "or_n3cfg", "or_n4cfg", "or_bintable3", "or_bintable4"):
@@ -128,20 +131,61 @@ def pcf2(xyz, w, n_data, p: Params, buf=None):
 
 
 def pcf3(xyz, w, n_data, p: Params, buf=None):
-    """Returns dict(cnt[ncfg,4] u64, n2[ncfg], n3[ncfg], n_cand)."""
+    """Returns dict(cnt u64, n2, n3, n_cand): cnt[ncfg,4], n2/n3[ncfg] for
+    nmu == 1; cnt[ncfg,nmu,4], n2/n3[ncfg,nmu] for the RSD 3PCF (nmu >= 2)."""
     xyz, w, buf = _prep(xyz, w, buf)
     n = xyz.shape[0]
     nc = n3cfg(p.nbins)
-    cnt = np.zeros((nc, 4), np.uint64)
-    n2 = np.zeros(nc)
-    n3 = np.zeros(nc)
+    cnt = np.zeros((nc, p.nmu, 4), np.uint64)
+    n2 = np.zeros((nc, p.nmu))
+    n3 = np.zeros((nc, p.nmu))
     cand = ctypes.c_uint64(0)
     rc = lib().or_3pcf(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
                        n_data, n - n_data, ctypes.byref(p.c()), _ptr(cnt, ctypes.c_uint64),
                        _ptr(n2, ctypes.c_double), _ptr(n3, ctypes.c_double), ctypes.byref(cand))
     if rc:
         raise ValueError(f"or_3pcf failed: {rc}")
+    if p.nmu == 1:
+        cnt, n2, n3 = cnt[:, 0], n2[:, 0], n3[:, 0]
     return dict(cnt=cnt, n2=n2, n3=n3, n_cand=int(cand.value))
+
+
+def normal_bin(nmu, mu1, mu2) -> int:
+    """find_normal: 1-based mu bin of a triplet from its two apex-side mu bins."""
+    return int(lib().or_normal_bin(nmu, mu1, mu2))
+
+
+def pcf3_equi(xyz, w, n_data, p: Params, buf=None):
+    """Equilateral 3PCF: dict(cnt[nbins,nmu,4] u64, n2[nbins,nmu], n3[nbins,nmu])."""
+    xyz, w, buf = _prep(xyz, w, buf)
+    n = xyz.shape[0]
+    cnt = np.zeros((p.nbins, p.nmu, 4), np.uint64)
+    n2 = np.zeros((p.nbins, p.nmu))
+    n3 = np.zeros((p.nbins, p.nmu))
+    rc = lib().or_3pcf_equi(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
+                            n_data, n - n_data, ctypes.byref(p.c()), _ptr(cnt, ctypes.c_uint64),
+                            _ptr(n2, ctypes.c_double), _ptr(n3, ctypes.c_double))
+    if rc:
+        raise ValueError(f"or_3pcf_equi failed: {rc}")
+    return dict(cnt=cnt, n2=n2, n3=n3)
+
+
+def pcf4_tetra(xyz, w, n_data, p: Params, buf=None):
+    """Regular-tetrahedron 4PCF: dict(cnt[nbins,16] u64, n2[nbins], n3[nbins],
+    disc_n[6,nbins], disc_r[6,nbins])."""
+    xyz, w, buf = _prep(xyz, w, buf)
+    n = xyz.shape[0]
+    nb = p.nbins
+    cnt = np.zeros((nb, 16), np.uint64)
+    n2, n3 = np.zeros(nb), np.zeros(nb)
+    dn, dr = np.zeros((6, nb)), np.zeros((6, nb))
+    rc = lib().or_4pcf_tetra(_ptr(xyz, ctypes.c_double), _ptr(w, ctypes.c_double), _ptr(buf, ctypes.c_int32),
+                             n_data, n - n_data, ctypes.byref(p.c()), _ptr(cnt, ctypes.c_uint64),
+                             _ptr(n2, ctypes.c_double), _ptr(n3, ctypes.c_double), _ptr(dn, ctypes.c_double),
+                             _ptr(dr, ctypes.c_double))
+    if rc:
+        raise ValueError(f"or_4pcf_tetra failed: {rc}")
+    return dict(cnt=cnt, n2=n2, n3=n3, disc_n=dn, disc_r=dr)
 
 
 def pcf4(xyz, w, n_data, p: Params, buf=None):

@@ -12,7 +12,8 @@ and manifest.json lists the parameters and the runs:
     T   -gal D+R as one file (N2 = (DD + DR + RD + RR) / N^2)
 The D/R/T runs recover exact integer counts per bin from the printed N2.
 
-Usage:  python tests/golden/make_golden.py     (needs the reference binary)
+Usage:  python tests/golden/make_golden.py [case ...]   (needs the reference binary;
+        with case names, only those cases are rewritten)
 """
 from __future__ import annotations
 
@@ -82,14 +83,54 @@ def case_defs():
     d = np.concatenate([d, d[:60]], axis=0)
     cases.append(dict(name="dups_rmin0", mode="-2pcf", d=d, r=None, wd=None, wr=None,
                       p=dict(rmin=0.0, rmax=5.0, nbins=5, nmu=1, log_bins=False), runs=["D"]))
+    # RSD 3PCF (-3pcf with nmu >= 2: find_normal mu bins)
+    d = C.clustered(1200, 80.0, 41, members=10, sigma=2.0)
+    r = C.uniform(2400, 80.0, 42)
+    cases.append(dict(name="p3_rsd", mode="-3pcf", d=d, r=r, wd=None, wr=None,
+                      p=dict(rmin=4.0, rmax=12.0, nbins=3, nmu=4, log_bins=False), runs=["DR", "D", "R", "T"]))
+    cases.append(dict(name="p3_rsd_w", mode="-3pcf", d=d, r=r,
+                      wd=C.lognormal_weights(1200, 0.2, 43), wr=C.lognormal_weights(2400, 0.2, 44),
+                      p=dict(rmin=4.0, rmax=12.0, nbins=3, nmu=5, log_bins=True), runs=["DR"]))
+    # equilateral 3PCF (-equi), isotropic and RSD
+    cases.append(dict(name="equi", mode="-equi", d=d, r=r, wd=None, wr=None,
+                      p=dict(rmin=4.0, rmax=12.0, nbins=4, nmu=1, log_bins=False), runs=["DR", "D", "R", "T"]))
+    cases.append(dict(name="equi_w", mode="-equi", d=d, r=r,
+                      wd=C.lognormal_weights(1200, 0.2, 43), wr=C.lognormal_weights(2400, 0.2, 44),
+                      p=dict(rmin=4.0, rmax=12.0, nbins=4, nmu=1, log_bins=True), runs=["DR"]))
+    cases.append(dict(name="equi_rsd", mode="-equi", d=d, r=r, wd=None, wr=None,
+                      p=dict(rmin=4.0, rmax=12.0, nbins=3, nmu=6, log_bins=False), runs=["DR", "D"]))
+    # regular-tetrahedron 4PCF (-tetra)
+    d = C.clustered(1000, 60.0, 51, members=10, sigma=2.0)
+    r = C.uniform(1000, 60.0, 52)
+    cases.append(dict(name="tetra", mode="-tetra", d=d, r=r, wd=None, wr=None,
+                      p=dict(rmin=3.0, rmax=8.0, nbins=3, nmu=1, log_bins=False), runs=["DR", "D", "R", "T"]))
+    cases.append(dict(name="tetra_w", mode="-tetra", d=d, r=r,
+                      wd=C.lognormal_weights(1000, 0.2, 53), wr=C.lognormal_weights(1000, 0.2, 54),
+                      p=dict(rmin=3.0, rmax=8.0, nbins=2, nmu=1, log_bins=False), runs=["DR"]))
+    # lattice: sides 1, sqrt(2), sqrt(3) on and around the bin edges
+    lat10b = C.lattice(10, 1.0, 0.0)   # the reference needs N >= 999 points (create_graph(1,999))
+    cases.append(dict(name="lattice_equi", mode="-equi", d=lat10b, r=None, wd=None, wr=None,
+                      p=dict(rmin=0.5, rmax=2.0, nbins=3, nmu=1, log_bins=False), runs=["D"]))
+    cases.append(dict(name="lattice_tetra", mode="-tetra", d=lat10b, r=None, wd=None, wr=None,
+                      p=dict(rmin=1.0, rmax=2.0, nbins=1, nmu=1, log_bins=False), runs=["D"]))
     return cases
 
 
 def main():
     if not os.path.exists(O.REF_BIN):
         sys.exit("reference binary missing: run `make -C oracle ref` first")
+    only = set(sys.argv[1:])
+    path = os.path.join(HERE, "manifest.json")
+    old = {}
+    if only and os.path.exists(path):
+        with open(path) as f:
+            old = {m["name"]: m for m in json.load(f)}
     manifest = []
     for cs in case_defs():
+        if only and cs["name"] not in only:
+            if cs["name"] in old:
+                manifest.append(old[cs["name"]])
+            continue
         cdir = os.path.join(HERE, cs["name"])
         os.makedirs(cdir, exist_ok=True)
         arrs = dict(xyz_d=cs["d"])
@@ -117,7 +158,7 @@ def main():
                              n_data=int(len(cs["d"])), n_rand=int(0 if cs["r"] is None else len(cs["r"])),
                              weighted=cs["wd"] is not None))
         print("wrote", cs["name"])
-    with open(os.path.join(HERE, "manifest.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(manifest, f, indent=1)
 
 

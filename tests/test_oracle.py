@@ -166,17 +166,23 @@ def test_3pcf_matches_reference_output(name):
     run = "DR" if "DR" in c["runs"] else "D"
     rows = parse_rows(read_text(name, run))
     res = O.pcf3(xyz, w, nd, p)
-    nc = O.n3cfg(p.nbins)
-    assert rows.shape == (nc, 9)
+    nc, nm = O.n3cfg(p.nbins), p.nmu
+    rsd = nm > 1
+    assert rows.shape == ((nc * nm, 11) if rsd else (nc, 9))
     edges = O.radial_bins(p)
     cfgs = [(i, j, k) for i in range(p.nbins) for j in range(i, p.nbins) for k in range(j, p.nbins)]
     ecols = np.array([[edges[i], edges[i + 1], edges[j], edges[j + 1], edges[k], edges[k + 1]] for i, j, k in cfgs])
+    ecols = np.repeat(ecols, nm, axis=0)
     assert close_e147(rows[:, :6], ecols).all()
-    assert close_e147(rows[:, 6], res["n2"]).all()
-    assert close_e147(rows[:, 7], res["n3"]).all()
-    cnt = res["cnt"]
-    assert (cnt % 3 == 0).all() or nr == 0 or True  # per-class counts need not be /3; totals are:
-    assert (cnt.sum(1) % 3 == 0).all()
+    col = 6
+    if rsd:   # mu columns (l-1)/nmu, l/nmu (gramsci.F90:532)
+        mu = np.tile(np.arange(nm), nc)
+        assert close_e147(rows[:, 6], mu / nm).all() and close_e147(rows[:, 7], (mu + 1) / nm).all()
+        col = 8
+    assert close_e147(rows[:, col], res["n2"].reshape(-1)).all()
+    assert close_e147(rows[:, col + 1], res["n3"].reshape(-1)).all()
+    cnt = res["cnt"].reshape(nc, nm, 4)
+    assert (cnt.sum((1, 2)) % 3 == 0).all()     # each triangle once per apex
 
 
 @pytest.mark.parametrize("name", [n for n in CASES_3 if not MAN[n]["weighted"]])
@@ -184,14 +190,42 @@ def test_3pcf_exact_counts_from_reference(name):
     c = MAN[name]
     p = _params(c)
     xyz, _, nd, nr = _cat(name)
-    cnt = O.pcf3(xyz, None, nd, p)["cnt"].astype(np.int64)
+    cnt = O.pcf3(xyz, None, nd, p)["cnt"].astype(np.int64).reshape(-1, 4)
     checks = {"D": (nd, cnt[:, 0]), "R": (nr, cnt[:, 3]), "T": (nd + nr, cnt.sum(1))}
+    col = 8 if p.nmu > 1 else 6
     for run, (n, expect) in checks.items():
         if run not in c["runs"]:
             continue
         rows = parse_rows(read_text(name, run))
-        got = np.rint(rows[:, 6] * float(n) ** 3).astype(np.int64)
+        got = np.rint(rows[:, col] * float(n) ** 3).astype(np.int64)
         assert np.array_equal(got, expect), (run, got, expect)
+
+
+def test_3pcf_rsd_sums_to_isotropic():
+    # the RSD split only redistributes the isotropic triplets over mu bins
+    name = "p3_rsd"
+    c = MAN[name]
+    p = _params(c)
+    xyz, _, nd, nr = _cat(name)
+    a = O.pcf3(xyz, None, nd, p)["cnt"].sum(1)
+    b = O.pcf3(xyz, None, nd, O.Params(p.rmin, p.rmax, p.nbins, 1, p.log_bins))["cnt"]
+    assert np.array_equal(a, b)
+
+
+def test_normal_bin_special_values():
+    # find_normal(mu1, mu2): the radicand 0.75 - a^2 - b^2 + ab of the bin centres
+    # a, b; at a = b = 0 (odd nmu, central bins) it is 0.75 and
+    # 1.1547 * sqrt(0.75) = 0.99999... < 1, so the top bin nmu is reached;
+    # a negative radicand (a = -b near +-1) lands in bin 1.
+    for nmu in (3, 5, 7, 127):
+        mid = (nmu + 1) // 2
+        assert O.normal_bin(nmu, mid, mid) == nmu
+        assert O.normal_bin(nmu, 1, nmu) == 1
+        assert O.normal_bin(nmu, nmu, 1) == 1
+    for nmu in range(2, 40):
+        for a in range(1, nmu + 1):
+            for b in range(1, nmu + 1):
+                assert 1 <= O.normal_bin(nmu, a, b) <= nmu
 
 
 def test_3pcf_permutation_invariant():

@@ -78,6 +78,8 @@ def lib():
     L.gs_count_2pcf.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
     L.gs_count_3pcf.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
     L.gs_count_4pcf.argtypes = [vp, pp, u64p, dp, dp, dp, dp, sp]
+    L.gs_count_3pcf_equi.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
+    L.gs_count_4pcf_tetra.argtypes = [vp, pp, u64p, dp, dp, dp, dp, sp]
     L.gs_pcf2.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, sp]
     L.gs_pcf3.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, sp]
     L.gs_pcf4.argtypes = [dp, dp, i32p, ctypes.c_int64, ctypes.c_int64, pp, u64p, dp, dp, dp, dp, sp]
@@ -85,6 +87,8 @@ def lib():
     L.gs_adjoint_2pcf.argtypes = [vp, pp, dp, dp, dp, sp]
     i64p = P(ctypes.c_int64)
     L.gs_set_catalogue_batched.argtypes = [vp, dp, dp, i32p, i64p, i64p, ctypes.c_int]
+    L.gs_decompose.argtypes = [vp, P(ctypes.c_float), i32p, ctypes.c_int64, ctypes.c_float, ctypes.c_int, i64p, i32p,
+                               i32p, ctypes.c_int64, P(ctypes.c_float)]
     L.gs_count_2pcf_batched.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
     L.gs_adjoint_2pcf_device.argtypes = [vp, pp, dp, dp, vp, sp]
     for name in exported_names():

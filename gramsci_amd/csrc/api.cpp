@@ -71,6 +71,7 @@ template struct DevBuf<float4>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<unsigned char>;
 template struct DevBuf<int64_t>;
+template struct DevBuf<float>;
 
 // ------------------------------------------------------------ thresholds
 // The reference's radial bin of a squared distance s (gramsci.F90:136-145,
@@ -124,7 +125,7 @@ int check_params(const gs_params* p, int mode)
     if (p->precision != GS_FP64 && p->precision != GS_FP32_GUARD)
         return fail(GS_EINVAL, "precision must be GS_FP64 or GS_FP32_GUARD");
     if (p->cell_div < 0 || p->cell_div > 16) return fail(GS_EINVAL, "cell_div must be in 0..16");
-    if (mode >= 3 && p->nmu != 1) return fail(GS_EINVAL, "3PCF/4PCF support nmu = 1 only");
+    if (mode >= 4 && p->nmu != 1) return fail(GS_EINVAL, "the 4PCF has no mu binning: nmu must be 1");
     return GS_OK;
 }
 
@@ -654,12 +655,12 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     a.n_chunks = c->n_chunks;
     a.reps = reps;
     a.mirror = (rsd && !c->has_buf) ? 1 : 0;
-    const int occ = pair2_occupancy(nb, nm, reps, fp64, rsd, weighted);
+    const int occ = pair2_occupancy(nb, nm, reps, fp64, rsd, weighted, p->log_bins != 0);
     int nblocks = c->n_cu * occ;
     if (nblocks > c->n_chunks) nblocks = std::max(c->n_chunks, 1);
     (void)hipEventRecord(c->ev[2], c->stream);
     if (c->n_chunks > 0) {
-        e = launch_pair2(a, nb, nm, fp64, rsd, weighted, nblocks, c->stream);
+        e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p->log_bins != 0, nblocks, c->stream);
         if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
     }
     (void)hipEventRecord(c->ev[3], c->stream);

@@ -170,7 +170,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
         pblob->release(); B.plans.pop_back();
         return fail(GS_ELIMIT, "per-wave histograms of %d bins do not fit in LDS", nbt);
     }
-    const int occ = pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, true);
+    const int occ = pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true);
     const int nblocks = v.n_cu * occ;
 
     // capture: clear, count sort into cells, scans, scatter, row chunks, pair kernel
@@ -188,7 +188,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     if (e == hipSuccess)
         e = launch_row_chunks_b(B.cell_start.ptr, d_reg, a.row_base, nreg, (int)rows_tot, B.nch.ptr, v.stream);
     if (e == hipSuccess) e = exclusive_scan(B.nch.ptr, B.row_chunk.ptr, rows_tot + 1, B.scan_tmp.ptr, v.stream);
-    if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, nblocks, v.stream);
+    if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p.log_bins != 0, nblocks, v.stream);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(v.stream, &graph);
     if (e == hipSuccess) e = e2;

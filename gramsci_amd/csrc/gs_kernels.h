@@ -26,6 +26,12 @@ hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* b
 hipError_t launch_row_chunks_b(const int32_t* start, const RegionDesc* reg, const int32_t* row_base, int nreg,
                                int nrow_tot, int32_t* nch, hipStream_t st);
 
+// domain decomposition (decompose.hip)
+hipError_t launch_region_box(const float* xyz, const int32_t* reg, int64_t n, int nreg, int32_t* keys, hipStream_t st);
+hipError_t launch_region_flag(const float* xyz, int64_t n, const float* ext, int32_t* flag, hipStream_t st);
+hipError_t launch_region_scatter(const float* xyz, int64_t n, const float* box, const int32_t* flag, const int32_t* pos,
+                                 int32_t* idx, int32_t* buf, hipStream_t st);
+
 // Arguments of the persistent 2PCF pair kernel (pair2.hip).
 struct PairArgs {
     const double4* pw;          // sorted points: x, y, z, raw weight
@@ -54,29 +60,35 @@ struct PairArgs {
 };
 
 // Arguments of the persistent 3PCF triplet kernel (triplet3.hip).
+// Histogram cell h = (cfg * nm + mu) * 4 + class.
 struct TripArgs {
     const double4* pw;
     const uint32_t* flags;
+    const int32_t* orig;        // original input index of each sorted point (RSD: find_normal argument order)
     const int32_t* cell_start;
     const Row* rows;            // full stencil
     const BinDesc* bins;
-    const uint16_t* cfg_tab;    // [nb][nb][nb] -> 0-based configuration
+    const uint16_t* cfg_tab;    // [nb][nb][nb] -> 0-based configuration (unused when equi)
+    const uint8_t* ntab;        // RSD: [nm][nm] 0-based find_normal bin of (mu(k1), mu(k2))
     int32_t* queue;             // 8 work counters
     int32_t* err;               // set to 1 if an apex has more than kmax neighbours
-    unsigned long long* out_cnt;    // [ncfg][4]
-    unsigned long long* out_w;      // [ncfg][4][2]
+    unsigned long long* out_cnt;    // [ncfg][nm][4]
+    unsigned long long* out_w;      // [ncfg][nm][4][2]
     unsigned long long* out_stats;  // candidates, rechecks, overflow, units, binned
     GridDesc g;
     int32_t nrows;
     int32_t n_items;            // apexes (all points; buffer points are skipped)
-    int32_t ncfg;
+    int32_t ncfg;               // configurations (nb for equi: the configuration is the common bin)
+    int32_t nm;                 // mu bins (1: isotropic)
+    int32_t equi;               // only triangles whose three sides share one bin
     int32_t reps;
     int32_t kmax;
+    int32_t lds_hist;           // histogram in LDS (else global atomics)
     int32_t pad;
 };
-size_t triplet3_lds_bytes(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted);
+size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);
-int triplet3_occupancy(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted);
+int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted);
 
 // Arguments of the persistent 4PCF quad kernel (quad4.hip).
 struct QuadArgs {
@@ -90,7 +102,7 @@ struct QuadArgs {
     int32_t* queue;
     int32_t* err;
     unsigned long long* out_cnt;    // [ncell][16]
-    double* out_w;              // weighted: [ncell][16] quad products, [6][ncell][4] disc pair products
+    double* out_w;              // weighted: [ncell][16] quad products, [6][ncell][4] disc pair (tetra: triple) products
     unsigned long long* out_stats;
     GridDesc g;
     int32_t nrows;
@@ -98,11 +110,11 @@ struct QuadArgs {
     int32_t ncell;
     int32_t kmax;
     int32_t lds_hist;           // counts in LDS (else global atomics)
-    int32_t pad;
+    int32_t tetra;              // regular tetrahedra only (all six sides in one bin; cell = that bin)
 };
-size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist);
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra);
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);
-int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool weighted);
+int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool weighted);
 
 // Arguments of the persistent 2PCF adjoint kernel (adjoint.hip).
 struct AdjArgs {
@@ -134,9 +146,9 @@ hipError_t launch_adj_finish(const double* G, int64_t n, int64_t n_data, double 
 
 // Dynamic LDS bytes of the pair kernel for these parameters.
 size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched = false);
-hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, int nblocks,
-                        hipStream_t st);
+hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, bool logb,
+                        int nblocks, hipStream_t st);
 // Work-groups per CU the pair kernel can keep resident for this LDS size.
-int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool batched = false);
+int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool logb, bool batched = false);
 
 }  // namespace gs

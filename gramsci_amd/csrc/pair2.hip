@@ -52,18 +52,19 @@ struct Stats {
     unsigned long long cand, rechk, over;
 };
 
+// One pair's histogram update.  base = class offset + b*nm (flat bin index
+// without the mu bin); counts live at ((base + m) << rsh) + rep.
 template <bool RSD, bool W>
-__device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw, int reps, int rep, int nb, int nm,
-                                           bool mirror, int cls, int b, int m, int mr, int ci, int cj, double p,
-                                           int shift, unsigned long long* out_cnt, unsigned long long* out_w)
+__device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw, int rsh, int rep, int nm,
+                                           bool mirror, int base, int m, int mr, int ci, int cj, double x,
+                                           unsigned long long* out_cnt, unsigned long long* out_w)
 {
-    const int base = (cls * nb + b) * nm;
     uint32_t qlo = 0, qhi = 0;
-    if constexpr (W) quantise(p, qlo, qhi);
+    if constexpr (W) quantise(x, qlo, qhi);
     if constexpr (!RSD) {
         const int c = ci + cj;
         if (c) {
-            atomicAdd(&hc[base * reps + rep], (uint32_t)c);
+            atomicAdd(&hc[(base << rsh) + rep], (uint32_t)c);
             if constexpr (W) {
                 atomicAdd(&hw[2 * base], (unsigned long long)qlo * c);
                 atomicAdd(&hw[2 * base + 1], (unsigned long long)qhi * c);
@@ -71,34 +72,36 @@ __device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw,
         }
     } else if (mirror && mr == nm - 1 - m) {
         // mirror-implied histogram: this update stands for i->j at m and j->i at nm-1-m
-        atomicAdd(&hc[(base + m) * reps + rep], 1u);
+        const int k = base + m;
+        atomicAdd(&hc[(k << rsh) + rep], 1u);
         if constexpr (W) {
-            atomicAdd(&hw[2 * (base + m)], (unsigned long long)qlo);
-            atomicAdd(&hw[2 * (base + m) + 1], (unsigned long long)qhi);
+            atomicAdd(&hw[2 * k], (unsigned long long)qlo);
+            atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi);
         }
     } else if (mirror) {
         // a pair exactly on a mu edge: both ordered pairs explicitly, straight to global memory (rare)
         const unsigned long long L = ((unsigned long long)qhi << 32) + qlo;
-        const unsigned long long H = (unsigned long long)qhi >> 32;
         atomicAdd(&out_cnt[base + m], 1ull);
         atomicAdd(&out_cnt[base + mr], 1ull);
         if constexpr (W) {
-            glb_add128(&out_w[2 * (base + m)], L, H);
-            glb_add128(&out_w[2 * (base + mr)], L, H);
+            glb_add128(&out_w[2 * (base + m)], L, 0ull);
+            glb_add128(&out_w[2 * (base + mr)], L, 0ull);
         }
     } else {
         if (ci) {
-            atomicAdd(&hc[(base + m) * reps + rep], 1u);
+            const int k = base + m;
+            atomicAdd(&hc[(k << rsh) + rep], 1u);
             if constexpr (W) {
-                atomicAdd(&hw[2 * (base + m)], (unsigned long long)qlo);
-                atomicAdd(&hw[2 * (base + m) + 1], (unsigned long long)qhi);
+                atomicAdd(&hw[2 * k], (unsigned long long)qlo);
+                atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi);
             }
         }
         if (cj) {
-            atomicAdd(&hc[(base + mr) * reps + rep], 1u);
+            const int k = base + mr;
+            atomicAdd(&hc[(k << rsh) + rep], 1u);
             if constexpr (W) {
-                atomicAdd(&hw[2 * (base + mr)], (unsigned long long)qlo);
-                atomicAdd(&hw[2 * (base + mr) + 1], (unsigned long long)qhi);
+                atomicAdd(&hw[2 * k], (unsigned long long)qlo);
+                atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi);
             }
         }
     }
@@ -150,7 +153,7 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool 
     return b;
 }
 
-template <bool FP64, bool RSD, bool W>
+template <bool FP64, bool RSD, bool W, bool LOGB>
 __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -164,6 +167,7 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     const int lane = tid & (kWave - 1);
     const int wv = tid / kWave;
     const int rep = lane & (reps - 1);
+    const int rsh = 31 - __clz(reps);
     const bool mirror = RSD && a.mirror;
     const int copies = batched ? kWG / kWave : 1;
 
@@ -197,9 +201,9 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     __syncthreads();
 
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi, ua = B.ua, ub = B.ub;
-    const bool logb = B.log_bins != 0;
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const double ms = B.mu_scale;
+    const int nbm = nb * nm;
     const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
     const int shift = B.fx_shift2;
     const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
@@ -257,7 +261,10 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
             fi = a.flags[i];
         }
         const int ri = fi & 1, ci = (fi & 2) ? 0 : 1;
+        const double wi = Pi.w;
         const float ax32 = (float)Pi.x, ay32 = (float)Pi.y, az32 = (float)Pi.z;   // absolute, for mu
+        // histogram class offsets of this lane's pairs with a data / a random neighbour
+        const int cb0 = ri ? nbm : 0, cb1 = cb0 + nbm;
 
         for (int r = 0; r < a.nrows; ++r) {
             const Row rw = a.rows[r];
@@ -290,35 +297,33 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                 }
                 wave_sync();
                 bound += (unsigned long long)nt * 2ull * kWave;
-                int kbeg = 0;
-                if (self) kbeg = max(0, i + 1 - jt);
-                if (!vi) kbeg = nt;
-                st.cand += (unsigned long long)max(0, nt - kbeg);
-                for (int k = kbeg; k < nt; ++k) {
+                // rotated schedule: at step k lane L takes tile entry (k + L) mod nt, so the
+                // lanes of a wave pair with different neighbours (spreading their histogram
+                // updates over many bins) while reading consecutive, conflict-free LDS words
+                const int kend = vi ? nt : 0;
+                const int lm = lane % nt;
+                const int jmin = self ? i + 1 - jt : 0;   // self row: only j > i
+                for (int k = 0; k < kend; ++k) {
+                    int kk = k + lm;
+                    kk = kk >= nt ? kk - nt : kk;
+                    if (kk < jmin) continue;
+                    ++st.cand;
                     int b, m = 0, mr = 0;
-                    const uint32_t fj = tf[k];
+                    const uint32_t fj = tf[kk];
                     if constexpr (FP64) {
-                        const double jx = tx[k], jy = ty[k], jz = tz[k];
+                        const double jx = tx[kk], jy = ty[kk], jz = tz[kk];
                         const double s = exact_s(Pi.x, Pi.y, Pi.z, jx, jy, jz);
                         if (!(s > t_lo && s < t_hi)) continue;
-                        const float sf = (float)s;
-                        const float u = logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub)
-                                             : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub);
-                        b = (int)u;
-                        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
-                        while (b > 0 && s < t64[b]) --b;
-                        while (b < nb && s >= t64[b + 1]) ++b;
+                        b = adjust_bin(s, bin_estimate((float)s, LOGB, ua, ub), t64, nb);
                         if (b == nb) { st.over += (uint64_t)(ci + ((fj & 2) ? 0 : 1)); continue; }
                         if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, jx, jy, jz, ms, nm, m, mr);
                     } else {
-                        const float4 tj = tl[k];
+                        const float4 tj = tl[kk];
                         const float dx = fmaf(tj.w - Li.w, h32, tj.x - Li.x);
                         const float dy = tj.y - yi_r, dz = tj.z - zi_r;
                         const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         if (!(s > rej_lo && s < rej_hi)) continue;
-                        const float u = logb ? fmaf(__builtin_amdgcn_logf(s), ua, ub)
-                                             : fmaf(__builtin_amdgcn_sqrtf(s), ua, ub);
-                        b = (int)u;
+                        b = (int)bin_estimate(s, LOGB, ua, ub);
                         b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
                         const float2 ok = okb[b];
                         bool safe = (s >= ok.x) && (s < ok.y);
@@ -328,22 +333,21 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                             const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
                             const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
                             const float rm = __builtin_amdgcn_rsqf(M2), rsq = __builtin_amdgcn_rsqf(s);
-                            const float th = D * rm * rsq;
-                            const float t = fmaf(th, ms32, ms32);
-                            const float tfl = floorf(t);
-                            const float fr = t - tfl;
+                            const float t = fmaf(D * rm * rsq, ms32, ms32);     // (theta + 1) * mu_scale
                             const float dt = ms32 * fmaf(mu_ew, rsq, fmaf(mu_em, rm, mu_c));
-                            const int mi = (int)tfl;
-                            const bool near_lo = (fr < dt) && (mi >= 1) && (mi <= nm - 1);
-                            const bool near_hi = (fr > 1.0f - dt) && (mi >= 0) && (mi <= nm - 2);
-                            // dt is not finite when mid or sep is ~0: always re-evaluate
-                            safe = safe && !near_lo && !near_hi && (dt < 0.25f);
+                            // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
+                            // nearest integer is no edge or is farther than the error bound dt < 1/2
+                            const float rt = rintf(t);
+                            const bool edge = (rt >= 1.0f) && (rt <= (float)(nm - 1));
+                            safe = safe && (dt < 0.5f) && (!edge || fabsf(t - rt) >= dt);
+                            const int mi = (int)floorf(t);
                             m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
                             mr = nm - 1 - m;
                         }
                         if (!safe) {
                             ++st.rechk;
-                            const double4 Pj = a.pw[jt + k];
+                            const double4 Pj = a.pw[jt + kk];
+                            Pi = a.pw[i];
                             const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
                             if (!(se > t_lo && se < t_hi)) continue;
                             b = exact_bin(se, t64, nb);
@@ -351,11 +355,11 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                             if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
                         }
                     }
-                    const int rj = fj & 1, cj = (fj & 2) ? 0 : 1;
-                    double pwt = 0.0;
-                    if constexpr (W) pwt = Pi.w * tw[k];
-                    accumulate<RSD, W>(hc, hw, reps, rep, nb, nm, mirror, ri + rj, b, m, mr, ci, cj, pwt, shift,
-                                       a.out_cnt, a.out_w);
+                    const int cj = (fj & 2) ? 0 : 1;
+                    const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
+                    double x = 0.0;
+                    if constexpr (W) x = wi * tw[kk];
+                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt, a.out_w);
                 }
                 // keep every u32 LDS counter (and the split sums) below 2^32: each wave drains
                 // after 2^29 of its own increments, so four waves stay below 2^31 in total
@@ -390,10 +394,10 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     }
 }
 
-template <bool FP64, bool RSD, bool W>
+template <bool FP64, bool RSD, bool W, bool LOGB>
 static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, hipStream_t st)
 {
-    auto kern = k_pair2<FP64, RSD, W>;
+    auto kern = k_pair2<FP64, RSD, W, LOGB>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -402,35 +406,53 @@ static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, int nblocks,
-                        hipStream_t st)
-{
-    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, a.reps, fp64, weighted, a.nreg > 0);
-    if (fp64) {
-        if (rsd) return weighted ? launch_t<true, true, true>(a, lds, nblocks, st) : launch_t<true, true, false>(a, lds, nblocks, st);
-        return weighted ? launch_t<true, false, true>(a, lds, nblocks, st) : launch_t<true, false, false>(a, lds, nblocks, st);
-    }
-    if (rsd) return weighted ? launch_t<false, true, true>(a, lds, nblocks, st) : launch_t<false, true, false>(a, lds, nblocks, st);
-    return weighted ? launch_t<false, false, true>(a, lds, nblocks, st) : launch_t<false, false, false>(a, lds, nblocks, st);
-}
-
-template <bool FP64, bool RSD, bool W>
+template <bool FP64, bool RSD, bool W, bool LOGB>
 static int occ_t(size_t lds)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pair2<FP64, RSD, W>, kWG, lds) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pair2<FP64, RSD, W, LOGB>, kWG, lds) != hipSuccess)
+        return 1;
     return nb > 0 ? nb : 1;
 }
 
-int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool batched)
+// Runs F<FP64, RSD, W, LOGB>(args...) for the runtime flags.
+template <template <bool, bool, bool, bool> class F, class... Args>
+static auto dispatch(bool fp64, bool rsd, bool w, bool logb, Args... args)
+{
+#define GS_D4(A, B, C, D) \
+    if (fp64 == A && rsd == B && w == C && logb == D) return F<A, B, C, D>::run(args...);
+    GS_D4(true, true, true, true) GS_D4(true, true, true, false) GS_D4(true, true, false, true)
+    GS_D4(true, true, false, false) GS_D4(true, false, true, true) GS_D4(true, false, true, false)
+    GS_D4(true, false, false, true) GS_D4(true, false, false, false) GS_D4(false, true, true, true)
+    GS_D4(false, true, true, false) GS_D4(false, true, false, true) GS_D4(false, true, false, false)
+    GS_D4(false, false, true, true) GS_D4(false, false, true, false) GS_D4(false, false, false, true)
+#undef GS_D4
+    return F<false, false, false, false>::run(args...);
+}
+
+template <bool A, bool B, bool C, bool D>
+struct Launch {
+    static hipError_t run(const PairArgs& a, size_t lds, int nblocks, hipStream_t st)
+    {
+        return launch_t<A, B, C, D>(a, lds, nblocks, st);
+    }
+};
+template <bool A, bool B, bool C, bool D>
+struct Occ {
+    static int run(size_t lds) { return occ_t<A, B, C, D>(lds); }
+};
+
+hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, bool logb,
+                        int nblocks, hipStream_t st)
+{
+    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, a.reps, fp64, weighted, a.nreg > 0);
+    return dispatch<Launch>(fp64, rsd, weighted, logb, a, lds, nblocks, st);
+}
+
+int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool logb, bool batched)
 {
     const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted, batched);
-    if (fp64) {
-        if (rsd) return weighted ? occ_t<true, true, true>(lds) : occ_t<true, true, false>(lds);
-        return weighted ? occ_t<true, false, true>(lds) : occ_t<true, false, false>(lds);
-    }
-    if (rsd) return weighted ? occ_t<false, true, true>(lds) : occ_t<false, true, false>(lds);
-    return weighted ? occ_t<false, false, true>(lds) : occ_t<false, false, false>(lds);
+    return dispatch<Occ>(fp64, rsd, weighted, logb, lds);
 }
 
 }  // namespace gs

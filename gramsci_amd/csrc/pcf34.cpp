@@ -1,4 +1,5 @@
-// pcf34.cpp -- C ABI for the 3PCF and 4PCF paths (gs_count_3pcf, gs_count_4pcf):
+// pcf34.cpp -- C ABI for the 3PCF and 4PCF paths (gs_count_3pcf, gs_count_3pcf_equi,
+// gs_count_4pcf, gs_count_4pcf_tetra):
 // grid, stencil, configuration tables, kernel launch and the host-side
 // combination of exact class counts / sums into the reference's N2, N3,
 // discN, discR.
@@ -75,17 +76,48 @@ static void fill_stats(gs_stats* st, hipEvent_t* ev, const unsigned long long* s
 
 // ====================================================================== 3PCF
 
-extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2, double* n3,
-                             gs_stats* st)
+namespace gs {
+
+// find_normal (gramsci.F90:1611-1625) tabulated over its two int8 arguments,
+// 0-based: mu11/mu22 are default reals (mu1-0.5 in single precision, the
+// division by the double mu_scale and "- 1.0" in double, rounded to single on
+// assignment), the rest of the expression is single precision left to right,
+// and floor()+1 is stored in an int8 before the clamp.  A negative radicand
+// (NaN) ends in bin 1 whatever integer its floor converts to.
+static std::vector<uint8_t> normal_table(int nm)
+{
+    std::vector<uint8_t> t((size_t)nm * nm, 0);
+    const double mu_scale = (double)(0.5f * (float)nm);
+    for (int a = 1; a <= nm; ++a)
+        for (int b = 1; b <= nm; ++b) {
+            const float x = (float)((double)((float)a - 0.5f) / mu_scale - 1.0);
+            const float y = (float)((double)((float)b - 0.5f) / mu_scale - 1.0);
+            volatile float r = 0.75f - x * x;
+            r = r - y * y;
+            r = r + x * y;
+            int v = 1;
+            if (r >= 0.0f) {
+                volatile float q = 1.1547f * std::sqrt((float)r);
+                q = q * (float)nm;
+                v = (int)(int8_t)((int)std::floor((float)q) + 1);
+                v = v < 1 ? 1 : (v > nm ? nm : v);
+            }
+            t[(size_t)(a - 1) * nm + (b - 1)] = (uint8_t)(v - 1);
+        }
+    return t;
+}
+
+// Shared body of gs_count_3pcf (all configurations) and gs_count_3pcf_equi.
+static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, double* wsum, double* n2, double* n3,
+                  gs_stats* st)
 {
     if (!c) return fail(GS_EINVAL, "NULL context");
     int rc = check_params(p, 3);
     if (rc) return rc;
     if ((rc = ctx_set_device(c))) return rc;
-    const int nb = p->nbins;
-    const int ncfg = nb * (nb + 1) * (nb + 2) / 6;
-    const int nh = ncfg * 4;
-    if (nh > 16384) return fail(GS_ELIMIT, "3PCF with %d bins has too many configurations for the LDS histogram", nb);
+    const int nb = p->nbins, nm = p->nmu;
+    const int ncfg = equi ? nb : nb * (nb + 1) * (nb + 2) / 6;
+    const int nh = ncfg * nm * 4;
     const bool fp64 = p->precision == GS_FP64;
     CtxView v = view(c);
     const bool weighted = v.has_w;
@@ -100,27 +132,44 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     std::vector<Row> rows = full_stencil(h, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
-    std::vector<uint16_t> tab = table3(nb);
+    std::vector<uint16_t> tab = equi ? std::vector<uint16_t>() : table3(nb);
+    std::vector<uint8_t> ntab = nm > 1 ? normal_table(nm) : std::vector<uint8_t>();
 
-    int reps = 1;
-    while (reps < 32 && (size_t)nh * reps * 2 * 4 <= 16 * 1024) reps *= 2;
-    int kmax = 4096;
+    TripArgs a{};
+    a.ncfg = ncfg;
+    a.nm = nm;
+    a.equi = equi ? 1 : 0;
+    // LDS plan: histogram replicas and neighbour capacity within 64 KiB; a
+    // histogram too large for LDS goes to global atomics instead
     const size_t budget = 64 * 1024;
-    while (kmax > 256 && triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted) > budget) kmax -= 256;
-    while (reps > 1 && triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted) > budget) reps /= 2;
-    if (triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted) > budget)
-        return fail(GS_ELIMIT, "3PCF tables do not fit in LDS");
+    auto lds = [&]() { return triplet3_lds_bytes(a, nb, fp64, weighted); };
+    a.lds_hist = 1;
+    a.reps = 1;
+    while (a.reps < 32 && (size_t)nh * a.reps * 2 * 4 <= 16 * 1024) a.reps *= 2;
+    a.kmax = 4096;
+    while (a.kmax > 512 && lds() > budget) a.kmax -= 256;
+    while (a.reps > 1 && lds() > budget) a.reps /= 2;
+    if (lds() > budget) {
+        a.lds_hist = 0;
+        a.reps = 1;
+        a.kmax = 4096;
+        while (a.kmax > 256 && lds() > budget) a.kmax -= 256;
+    }
+    if (lds() > budget) return fail(GS_ELIMIT, "3PCF tables do not fit in LDS");
 
     GsExtra& X = *v.extra;
     const size_t tab_off = ((sizeof(BinDesc) + 15) / 16) * 16;
-    const size_t rows_off = tab_off + ((tab.size() * 2 + 15) / 16) * 16;
+    const size_t ntab_off = tab_off + ((tab.size() * 2 + 15) / 16) * 16;
+    const size_t rows_off = ntab_off + ((ntab.size() + 15) / 16) * 16;
     const size_t nout = (size_t)nh + 2 * (size_t)nh + 8;
     if ((rc = X.params.reserve(rows_off + rows.size() * sizeof(Row) + 16)) || (rc = X.out.reserve(nout)) ||
         (rc = X.queue.reserve(16)))
         return rc;
     hipError_t e = hipMemcpyAsync(X.params.ptr, &B, sizeof(B), hipMemcpyHostToDevice, v.stream);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !tab.empty())
         e = hipMemcpyAsync(X.params.ptr + tab_off, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess && !ntab.empty())
+        e = hipMemcpyAsync(X.params.ptr + ntab_off, ntab.data(), ntab.size(), hipMemcpyHostToDevice, v.stream);
     if (e == hipSuccess && !rows.empty())
         e = hipMemcpyAsync(X.params.ptr + rows_off, rows.data(), rows.size() * sizeof(Row), hipMemcpyHostToDevice,
                            v.stream);
@@ -128,13 +177,14 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     if (e == hipSuccess) e = hipMemsetAsync(X.queue.ptr, 0, 16 * sizeof(int32_t), v.stream);
     if (e != hipSuccess) return hip_fail(e, "3PCF parameter upload");
 
-    TripArgs a{};
     a.pw = v.pw;
     a.flags = v.flags;
+    a.orig = v.orig;
     a.cell_start = v.cell_start;
     a.rows = (const Row*)(X.params.ptr + rows_off);
     a.bins = (const BinDesc*)X.params.ptr;
-    a.cfg_tab = (const uint16_t*)(X.params.ptr + tab_off);
+    a.cfg_tab = tab.empty() ? nullptr : (const uint16_t*)(X.params.ptr + tab_off);
+    a.ntab = ntab.empty() ? nullptr : (const uint8_t*)(X.params.ptr + ntab_off);
     a.queue = X.queue.ptr;
     a.err = X.queue.ptr + 8;
     a.out_cnt = X.out.ptr;
@@ -144,10 +194,7 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     a.g.m = mst;
     a.nrows = (int)rows.size();
     a.n_items = (int)(v.n_data + v.n_rand);
-    a.ncfg = ncfg;
-    a.reps = reps;
-    a.kmax = kmax;
-    const int occ = triplet3_occupancy(nb, ncfg, reps, kmax, fp64, weighted);
+    const int occ = triplet3_occupancy(a, nb, fp64, weighted);
     int nblocks = std::max(1, std::min(v.n_cu * occ, a.n_items));
     (void)hipEventRecord(v.ev[2], v.stream);
     if (a.n_items > 0) {
@@ -161,7 +208,7 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     if (e == hipSuccess) e = hipMemcpyAsync(&err, X.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
     if (e != hipSuccess) return hip_fail(e, "3PCF kernel");
-    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", kmax);
+    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", a.kmax);
 
     std::vector<long double> S(nh);
     for (int k = 0; k < nh; ++k) {
@@ -169,9 +216,11 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
         S[k] = weighted ? fx_value(h_out[nh + 2 * k], h_out[nh + 2 * k + 1], B.fx_shift3) : (long double)h_out[k];
         if (wsum) wsum[k] = (double)S[k];
     }
-    // N2 = sum over classes of (-1)^r S_r / (Wd^(3-r) Wr^r); N3 = S_3 / Wr^3 (gramsci.F90:505-507)
+    // N2 = sum over classes of (-1)^r S_r / (Wd^(3-r) Wr^r) (gramsci.F90:501, :507, :431, :437);
+    // N3 = S_3 / Wr^3 (:498, :505, :434) -- the equilateral RSD branch has no N3 (:429-431)
     const long double Wd = v.wsum_d, Wr = v.wsum_r;
-    for (int q = 0; q < ncfg; ++q) {
+    const bool with_n3 = !(equi && nm > 1);
+    for (int q = 0; q < ncfg * nm; ++q) {
         long double v2 = 0, v3 = 0;
         for (int r = 0; r <= 3; ++r) {
             const long double s = S[q * 4 + r];
@@ -181,7 +230,7 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
             for (int t = 0; t < r; ++t) den *= Wr;
             v2 += ((r & 1) ? -s : s) / den;
         }
-        if (Wr > 0) v3 = S[q * 4 + 3] / (Wr * Wr * Wr);
+        if (with_n3 && Wr > 0) v3 = S[q * 4 + 3] / (Wr * Wr * Wr);
         if (n2) n2[q] = (double)v2;
         if (n3) n3[q] = (double)v3;
     }
@@ -189,19 +238,36 @@ extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     return GS_OK;
 }
 
+}  // namespace gs
+
+extern "C" int gs_count_3pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2, double* n3,
+                             gs_stats* st)
+{
+    return count3(c, p, false, cnt, wsum, n2, n3, st);
+}
+
+extern "C" int gs_count_3pcf_equi(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2,
+                                  double* n3, gs_stats* st)
+{
+    return count3(c, p, true, cnt, wsum, n2, n3, st);
+}
+
 // ====================================================================== 4PCF
 
-extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* n2, double* n3, double* disc_n,
-                             double* disc_r, gs_stats* st)
+namespace gs {
+
+// Shared body of gs_count_4pcf (perimeter configurations) and gs_count_4pcf_tetra.
+static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, double* n2, double* n3, double* disc_n,
+                  double* disc_r, gs_stats* st)
 {
     if (!c) return fail(GS_EINVAL, "NULL context");
     int rc = check_params(p, 4);
     if (rc) return rc;
     if ((rc = ctx_set_device(c))) return rc;
     const int nb = p->nbins;
-    if (nb > 16) return fail(GS_ELIMIT, "4PCF supports nbins <= 16 (got %d)", nb);
-    const int ncfg = nb * (nb + 1) * (nb + 2) * (nb + 3) / 24;
-    const int ncell = ncfg * nb * nb;
+    if (!tetra && nb > 16) return fail(GS_ELIMIT, "4PCF supports nbins <= 16 (got %d)", nb);
+    const int ncfg = tetra ? nb : nb * (nb + 1) * (nb + 2) * (nb + 3) / 24;
+    const int ncell = tetra ? nb : ncfg * nb * nb;
     CtxView v = view(c);
     const bool weighted = v.has_w;
     const int m = p->cell_div > 0 ? p->cell_div : 2;
@@ -215,18 +281,19 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     std::vector<Row> rows = full_stencil(h, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
-    std::vector<uint16_t> tab = table4(nb);
+    std::vector<uint16_t> tab = tetra ? std::vector<uint16_t>() : table4(nb);
 
     const size_t budget = 64 * 1024;
     int kmax = 256;
     bool lds_hist = (size_t)ncell * 64 <= 16 * 1024;
-    while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) kmax /= 2;
-    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) {
+    while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) kmax /= 2;
+    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) {
         lds_hist = false;
         kmax = 256;
-        while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) kmax /= 2;
+        while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) kmax /= 2;
     }
-    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist) > budget) return fail(GS_ELIMIT, "4PCF tables do not fit in LDS");
+    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget)
+        return fail(GS_ELIMIT, "4PCF tables do not fit in LDS");
 
     GsExtra& X = *v.extra;
     const size_t tab_off = ((sizeof(BinDesc) + 15) / 16) * 16;
@@ -238,7 +305,7 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
         (rc = X.queue.reserve(16)) || (rc = X.dbl.reserve(std::max<size_t>(nw, 1))))
         return rc;
     hipError_t e = hipMemcpyAsync(X.params.ptr, &B, sizeof(B), hipMemcpyHostToDevice, v.stream);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !tab.empty())
         e = hipMemcpyAsync(X.params.ptr + tab_off, tab.data(), tab.size() * 2, hipMemcpyHostToDevice, v.stream);
     if (e == hipSuccess && !rows.empty())
         e = hipMemcpyAsync(X.params.ptr + rows_off, rows.data(), rows.size() * sizeof(Row), hipMemcpyHostToDevice,
@@ -255,7 +322,7 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     a.cell_start = v.cell_start;
     a.rows = (const Row*)(X.params.ptr + rows_off);
     a.bins = (const BinDesc*)X.params.ptr;
-    a.cfg_tab = (const uint16_t*)(X.params.ptr + tab_off);
+    a.cfg_tab = tab.empty() ? nullptr : (const uint16_t*)(X.params.ptr + tab_off);
     a.queue = X.queue.ptr;
     a.err = X.queue.ptr + 8;
     a.out_cnt = X.out.ptr;
@@ -268,7 +335,8 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     a.ncell = ncell;
     a.kmax = kmax;
     a.lds_hist = lds_hist ? 1 : 0;
-    const int occ = quad4_occupancy(nb, ncell, kmax, lds_hist, weighted);
+    a.tetra = tetra ? 1 : 0;
+    const int occ = quad4_occupancy(nb, ncell, kmax, lds_hist, tetra, weighted);
     int nblocks = std::max(1, std::min(v.n_cu * occ, a.n_items));
     (void)hipEventRecord(v.ev[2], v.stream);
     if (a.n_items > 0) {
@@ -305,7 +373,10 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     }
     // disc terms: t -> (points a, b of the pair product; point c that must be random)
     //   t0: (i,k1 | k3)  t1: (i,k2 | k3)  t2: (i,k3 | k1)  t3: (k1,k2 | i)  t4: (k3,k2 | i)  t5: (k3,k1 | i)
+    // the tetrahedron's terms are triple products that also carry c's normalised
+    // weight, -w_c/Wr (gramsci.F90:590-631)
     static const int ta[6] = {0, 0, 0, 1, 3, 3}, tb[6] = {1, 2, 3, 2, 2, 1}, tc[6] = {3, 3, 1, 0, 0, 0};
+    const long double fc = tetra ? norm(1) : 1.0L;
     for (int t = 0; t < 6; ++t)
         for (int q = 0; q < ncell; ++q) {
             long double dn = 0, dr = 0;
@@ -321,7 +392,7 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
                             s += (long double)h_out[(size_t)q * 16 + pat];
                 }
                 if (s == 0) continue;
-                const long double term = s * norm(ra) * norm(rb);
+                const long double term = s * norm(ra) * norm(rb) * fc;
                 dn += term;
                 if (ra && rb) dr += term;
             }
@@ -332,4 +403,18 @@ extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     for (int k = 0; k < 5; ++k) sts[k] = h_out[ncnt + k];
     fill_stats(st, v.ev, sts.data(), mst);
     return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" int gs_count_4pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* n2, double* n3, double* disc_n,
+                             double* disc_r, gs_stats* st)
+{
+    return count4(c, p, false, cnt, n2, n3, disc_n, disc_r, st);
+}
+
+extern "C" int gs_count_4pcf_tetra(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* n2, double* n3,
+                                   double* disc_n, double* disc_r, gs_stats* st)
+{
+    return count4(c, p, true, cnt, n2, n3, disc_n, disc_r, st);
 }

@@ -19,6 +19,11 @@
 // Counts go to an LDS histogram [cell][16 random patterns] (or straight to
 // global memory when it does not fit); weighted runs add FP64 sums of the
 // quad products per pattern and of the six disc pair products (:754-818).
+//
+// With `tetra` the kernel is query_graph_tetrahedron (:548-660) instead: no
+// minimum neighbour count, only quads whose six sides share one bin b
+// (cell = b), and the disc sums are triple products that include the weight
+// of the point whose random status selects the term (:590-631).
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -26,12 +31,12 @@ namespace gs {
 
 using namespace dev;
 
-size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist)
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra)
 {
     size_t b = 0;
     if (lds_hist) b += (size_t)ncell * 16 * 4;
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;            // t64
-    b += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;          // perimeter table (u16)
+    if (!tetra) b += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;   // perimeter table (u16)
     b += (size_t)kmax * (24 + 8 + 4 + 4 + 4);                     // x, y, z, w, key, slot/packed, sorted j
     b += ((size_t)kmax * kmax + 15) / 16 * 16;                    // link matrix (u8: bin+1, 0 = not kept)
     return b;
@@ -46,6 +51,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     const int kmax = a.kmax;
     const int ncell = a.ncell;
     const bool lh = a.lds_hist != 0;
+    const bool tetra = a.tetra != 0;
     const int tid = threadIdx.x;
 
     unsigned char* p = smem;
@@ -54,7 +60,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
     uint16_t* tab = (uint16_t*)p;
-    p += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;
+    if (!tetra) p += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;
     double* nx_ = (double*)p;
     double* ny_ = nx_ + kmax;
     double* nz_ = ny_ + kmax;
@@ -69,7 +75,8 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     if (lh)
         for (int k = tid; k < ncell * 16; k += kWG) hc[k] = 0;
     for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
-    for (int k = tid; k < nb * nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
+    if (!tetra)
+        for (int k = tid; k < nb * nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
     __syncthreads();
 
     const int nxg = a.g.dim[0], nyg = a.g.dim[1], nzg = a.g.dim[2];
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         __syncthreads();
         const int K = min(s_k, kmax);
         if (s_over && tid == 0) atomicExch(a.err, 1);
-        if (K <= 3) {        // gramsci.F90:675
+        if (K <= (tetra ? 2 : 3)) {        // gramsci.F90:675 (the tetrahedron query has no such skip)
             __syncthreads();
             continue;
         }
@@ -187,10 +194,10 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         for (long long pi = tid; pi < P; pi += kWG) {
             const int v = u + 1 + (int)rem;
             const int ind4 = L[u * K + v];
-            if (ind4) {
+            const int p1 = pk[u], p2 = pk[v];
+            const int ind1 = (p1 & 255) + 1, ind2 = (p2 & 255) + 1;
+            if (ind4 && (!tetra || (ind2 == ind1 && ind4 == ind1))) {     // tetra: gramsci.F90:564-567
                 units += (unsigned long long)(K - 1 - v);
-                const int p1 = pk[u], p2 = pk[v];
-                const int ind1 = (p1 & 255) + 1, ind2 = (p2 & 255) + 1;
                 const int r1 = (p1 >> 8) & 1, r2 = (p2 >> 8) & 1;
                 for (int k3 = v + 1; k3 < K; ++k3) {
                     const int ind5 = L[v * K + k3];
@@ -199,10 +206,16 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
                     if (!ind6) continue;
                     const int p3 = pk[k3];
                     const int ind3 = (p3 & 255) + 1;
-                    const int cfg = tab[(((ind1 - 1) * nb + (ind4 - 1)) * nb + (ind5 - 1)) * nb + (ind3 - 1)];
-                    if (cfg == 0) continue;
+                    int cell;
+                    if (tetra) {
+                        if (ind3 != ind1 || ind5 != ind1 || ind6 != ind1) continue;   // :571-579
+                        cell = ind1 - 1;
+                    } else {
+                        const int cfg = tab[(((ind1 - 1) * nb + (ind4 - 1)) * nb + (ind5 - 1)) * nb + (ind3 - 1)];
+                        if (cfg == 0) continue;
+                        cell = (cfg - 1) * nb2 + (ind2 - 1) * nb + (ind6 - 1);
+                    }
                     const int r3 = (p3 >> 8) & 1;
-                    const int cell = (cfg - 1) * nb2 + (ind2 - 1) * nb + (ind6 - 1);
                     const int pat = ri | (r1 << 1) | (r2 << 2) | (r3 << 3);
                     ++binned;
                     if (lh) atomicAdd(&hc[cell * 16 + pat], 1u);
@@ -212,17 +225,19 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
                         double* ow = a.out_w;
                         atomicAdd(&ow[(size_t)cell * 16 + pat], wi * w1 * w2 * w3);   // gramsci.F90:710
                         double* od = ow + (size_t)ncell * 16;                       // [6][ncell][4]
-                        // disc pair products and their conditions, gramsci.F90:754-818
+                        // disc pair products and their conditions, gramsci.F90:754-818; the
+                        // tetrahedron's terms also carry the conditioning point's weight (:590-631)
+                        const double fi = tetra ? wi : 1.0, f1 = tetra ? w1 : 1.0, f3 = tetra ? w3 : 1.0;
                         if (ri) {
-                            atomicAdd(&od[((size_t)3 * ncell + cell) * 4 + (r1 | r2 << 1)], w1 * w2);
-                            atomicAdd(&od[((size_t)4 * ncell + cell) * 4 + (r3 | r2 << 1)], w3 * w2);
-                            atomicAdd(&od[((size_t)5 * ncell + cell) * 4 + (r3 | r1 << 1)], w3 * w1);
+                            atomicAdd(&od[((size_t)3 * ncell + cell) * 4 + (r1 | r2 << 1)], w1 * w2 * fi);
+                            atomicAdd(&od[((size_t)4 * ncell + cell) * 4 + (r3 | r2 << 1)], w3 * w2 * fi);
+                            atomicAdd(&od[((size_t)5 * ncell + cell) * 4 + (r3 | r1 << 1)], w3 * w1 * fi);
                         }
                         if (r3) {
-                            atomicAdd(&od[((size_t)1 * ncell + cell) * 4 + (ri | r2 << 1)], wi * w2);
-                            atomicAdd(&od[((size_t)0 * ncell + cell) * 4 + (ri | r1 << 1)], wi * w1);
+                            atomicAdd(&od[((size_t)1 * ncell + cell) * 4 + (ri | r2 << 1)], wi * w2 * f3);
+                            atomicAdd(&od[((size_t)0 * ncell + cell) * 4 + (ri | r1 << 1)], wi * w1 * f3);
                         }
-                        if (r1) atomicAdd(&od[((size_t)2 * ncell + cell) * 4 + (ri | r3 << 1)], wi * w3);
+                        if (r1) atomicAdd(&od[((size_t)2 * ncell + cell) * 4 + (ri | r3 << 1)], wi * w3 * f1);
                     }
                 }
             }
@@ -263,13 +278,13 @@ static hipError_t launch_q(const QuadArgs& a, size_t lds, int nblocks, hipStream
 
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st)
 {
-    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0);
+    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0);
     return weighted ? launch_q<true>(a, lds, nblocks, st) : launch_q<false>(a, lds, nblocks, st);
 }
 
-int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool weighted)
+int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool weighted)
 {
-    const size_t lds = quad4_lds_bytes(nb, ncell, kmax, lds_hist);
+    const size_t lds = quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra);
     int n = 0;
     hipError_t e = weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_quad4<true>, kWG, lds)
                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_quad4<false>, kWG, lds);

@@ -1,4 +1,4 @@
-// triplet3.hip -- persistent 3PCF triplet kernel (all configurations).
+// triplet3.hip -- persistent 3PCF triplet kernel.
 //
 // Replaces create_graph (gramsci.F90:288-348) + query_graph_3pcf_all
 // (:463-545) with find_dist (:1579-1609).  For every apex i that is not a
@@ -7,6 +7,14 @@
 // configuration of its sorted side bins (bintable, :1459-1474) and in the
 // class given by the number of randoms among (i, k1, k2).  Each triangle is
 // thus counted once per apex (3 times in total), as in the reference.
+//
+// Variants selected at run time:
+//   nm >= 2  RSD 3PCF (:495-500): the triangle also goes to the mu bin
+//            find_normal(mu(k1), mu(k2)) (:1611-1625), tabulated on the host;
+//            mu(k) is the apex -> k mu bin of create_graph (:320-328), k1 the
+//            neighbour with the smaller original index;
+//   equi     query_graph_equilateral_triangle (:402-461): only triangles whose
+//            three sides share one bin b, counted at configuration b.
 //
 // One work-group per apex, persistent over apexes in cell order (8 XCD
 // queues as in the pair kernel):
@@ -25,62 +33,81 @@ namespace gs {
 
 using namespace dev;
 
-size_t triplet3_lds_bytes(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted)
+size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
 {
+    const size_t nh = (size_t)a.ncfg * a.nm * 4;
+    const size_t kmax = (size_t)a.kmax;
     size_t b = 0;
-    const size_t nh = (size_t)ncfg * 4;
-    b += (nh * reps * 4 + 15) / 16 * 16;                       // counts
-    if (weighted) b += nh * 16;                                 // split fixed-point sums
+    if (a.lds_hist) {
+        b += (nh * a.reps * 4 + 15) / 16 * 16;                  // counts
+        if (weighted) b += nh * 16;                             // split fixed-point sums
+    }
     b += ((size_t)(kMaxBins + 1) * 8);                          // ok_lo/ok_hi
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;           // t64
     b += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;             // config table (u16)
-    const size_t rec = fp64 ? 32 : 16;                          // coords + packed bin/class
-    b += (size_t)kmax * (rec + 4 + (weighted ? 8 : 0));         // + sorted index + weight
+    if (a.nm > 1) b += ((size_t)a.nm * a.nm + 15) / 16 * 16;    // find_normal table (u8)
+    const size_t rec = fp64 ? 32 : 16;                          // coords + packed bin/class/mu
+    b += kmax * (rec + 4 + (weighted ? 8 : 0) + (a.nm > 1 ? 4 : 0));   // + sorted index, weight, orig
     return b;
 }
 
-// record packing: bits 0..7 bin (0-based), bit 8 random
+// record packing: bits 0..7 bin (0-based), bit 8 random, bits 9..15 mu bin (0-based)
 template <bool FP64, bool W>
 __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const BinDesc& B = *a.bins;
     const int nb = B.nb;
-    const int ncfg = a.ncfg;
-    const int nh = ncfg * 4;
+    const int nm = a.nm;
+    const int nh = a.ncfg * nm * 4;
     const int reps = a.reps;
     const int kmax = a.kmax;
+    const bool lh = a.lds_hist != 0;
+    const bool equi = a.equi != 0;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int rep = lane & (reps - 1);
 
     unsigned char* p = smem;
     uint32_t* hc = (uint32_t*)p;
-    p += ((size_t)nh * reps * 4 + 15) / 16 * 16;
-    unsigned long long* hw = (unsigned long long*)p;
-    if (W) p += (size_t)nh * 16;
+    unsigned long long* hw = nullptr;
+    if (lh) {
+        p += ((size_t)nh * reps * 4 + 15) / 16 * 16;
+        hw = (unsigned long long*)p;
+        if (W) p += (size_t)nh * 16;
+    }
     float2* okb = (float2*)p;
     p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
     uint16_t* tab = (uint16_t*)p;
     p += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;
-    float4* rec32 = (float4*)p;          // FP32: x, y, z relative to the apex, packed bin/class
-    double4* rec64 = (double4*)p;        // FP64: absolute x, y, z, packed bin/class (as a double)
+    uint8_t* ntab = (uint8_t*)p;
+    if (nm > 1) p += ((size_t)nm * nm + 15) / 16 * 16;
+    float4* rec32 = (float4*)p;          // FP32: x, y, z relative to the apex, packed bin/class/mu
+    double4* rec64 = (double4*)p;        // FP64: absolute x, y, z, packed bin/class/mu (as a double)
     p += (size_t)kmax * (FP64 ? 32 : 16);
     int32_t* jl = (int32_t*)p;
     p += (size_t)kmax * 4;
     double* wl = (double*)p;             // neighbour weights * 2^shift3
+    if (W) p += (size_t)kmax * 8;
+    int32_t* ol = (int32_t*)p;           // RSD: original index of each neighbour
 
     __shared__ int s_item, s_k, s_over;
 
-    for (int k = tid; k < nh * reps; k += kWG) hc[k] = 0;
-    if (W)
-        for (int k = tid; k < 2 * nh; k += kWG) hw[k] = 0;
+    if (lh) {
+        for (int k = tid; k < nh * reps; k += kWG) hc[k] = 0;
+        if (W)
+            for (int k = tid; k < 2 * nh; k += kWG) hw[k] = 0;
+    }
     for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
     for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
-    for (int k = tid; k < nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
+    if (!equi)
+        for (int k = tid; k < nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
+    if (nm > 1)
+        for (int k = tid; k < nm * nm; k += kWG) ntab[k] = a.ntab[k];
     __syncthreads();
+    const double ms = B.mu_scale;
 
     const float ua = B.ua, ub = B.ub;
     const bool logb = B.log_bins != 0;
@@ -138,7 +165,13 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const int pos = atomicAdd(&s_k, 1);
                 if (pos >= kmax) { s_over = 1; continue; }
                 const uint32_t fj = a.flags[j];
-                const uint32_t pk = (uint32_t)b | ((fj & 1u) << 8);
+                uint32_t pk = (uint32_t)b | ((fj & 1u) << 8);
+                if (nm > 1) {
+                    int m, mr;
+                    exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);   // apex -> j
+                    pk |= (uint32_t)m << 9;
+                    ol[pos] = a.orig[j];
+                }
                 if constexpr (FP64) {
                     rec64[pos] = make_double4(Pj.x, Pj.y, Pj.z, (double)pk);
                 } else {
@@ -172,14 +205,16 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const double4 Ru = rec64[u], Rv = rec64[v];
                 pu = (uint32_t)Ru.w;
                 pv = (uint32_t)Rv.w;
-                const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
-                if (s > t_lo && s < t_hi) {
-                    const float sf = (float)s;
-                    int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub));
-                    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
-                    while (b > 0 && s < t64[b]) --b;
-                    while (b < nb && s >= t64[b + 1]) ++b;
-                    if (b == nb) ++over; else b3 = b;
+                if (!equi || ((pu ^ pv) & 255u) == 0) {
+                    const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
+                    if (s > t_lo && s < t_hi) {
+                        const float sf = (float)s;
+                        int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub));
+                        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                        while (b > 0 && s < t64[b]) --b;
+                        while (b < nb && s >= t64[b + 1]) ++b;
+                        if (b == nb) ++over; else b3 = b;
+                    }
                 }
             } else {
                 const float4 Ru = rec32[u], Rv = rec32[v];
@@ -187,7 +222,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 pv = __float_as_uint(Rv.w);
                 const float dx = Rv.x - Ru.x, dy = Rv.y - Ru.y, dz = Rv.z - Ru.z;
                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if (s > rej_lo && s < rej_hi) {
+                if ((!equi || ((pu ^ pv) & 255u) == 0) && s > rej_lo && s < rej_hi) {
                     int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(s), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(s), ua, ub));
                     b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
                     const float2 ok = okb[b];
@@ -204,17 +239,32 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     }
                 }
             }
+            if (equi && b3 != (int)(pu & 255u)) b3 = -1;
             if (b3 >= 0) {
-                const int cfg = tab[((pu & 255u) * nb + (pv & 255u)) * nb + b3];
+                const int cfg = equi ? b3 : (int)tab[((pu & 255u) * nb + (pv & 255u)) * nb + b3];
+                int mu = 0;
+                if (nm > 1) {
+                    const int mu_u = (int)((pu >> 9) & 127u), mu_v = (int)((pv >> 9) & 127u);
+                    mu = ol[u] < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
+                }
                 const int cls = ra + (int)((pu >> 8) & 1u) + (int)((pv >> 8) & 1u);
-                const int hk = cfg * 4 + cls;
-                atomicAdd(&hc[hk * reps + rep], 1u);
+                const int hk = (cfg * nm + mu) * 4 + cls;
                 ++binned;
-                if constexpr (W) {
-                    uint32_t qlo, qhi;
-                    quantise((wa * ldexp(wl[u], -shift)) * wl[v], qlo, qhi);
-                    atomicAdd(&hw[2 * hk], (unsigned long long)qlo);
-                    atomicAdd(&hw[2 * hk + 1], (unsigned long long)qhi);
+                if (lh) {
+                    atomicAdd(&hc[hk * reps + rep], 1u);
+                    if constexpr (W) {
+                        uint32_t qlo, qhi;
+                        quantise((wa * ldexp(wl[u], -shift)) * wl[v], qlo, qhi);
+                        atomicAdd(&hw[2 * hk], (unsigned long long)qlo);
+                        atomicAdd(&hw[2 * hk + 1], (unsigned long long)qhi);
+                    }
+                } else {
+                    atomicAdd(&a.out_cnt[hk], 1ull);
+                    if constexpr (W) {
+                        uint32_t qlo, qhi;
+                        quantise((wa * ldexp(wl[u], -shift)) * wl[v], qlo, qhi);
+                        glb_add128(&a.out_w[2 * hk], ((unsigned long long)qhi << 32) + qlo, 0ull);
+                    }
                 }
             }
             rem += kWG;
@@ -222,7 +272,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         }
         bound += (unsigned long long)P;
         __syncthreads();
-        if (bound > (1ull << 31)) {
+        if (lh && bound > (1ull << 31)) {
             for (int k = tid; k < nh; k += kWG) {
                 unsigned long long s = 0;
                 for (int r = 0; r < reps; ++r) { s += hc[k * reps + r]; hc[k * reps + r] = 0; }
@@ -242,7 +292,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         }
     }
     __syncthreads();
-    for (int k = tid; k < nh; k += kWG) {
+    for (int k = tid; lh && k < nh; k += kWG) {
         unsigned long long s = 0;
         for (int r = 0; r < reps; ++r) s += hc[k * reps + r];
         if (s) atomicAdd(&a.out_cnt[k], s);
@@ -285,7 +335,7 @@ static hipError_t launch_t3(const TripArgs& a, size_t lds, int nblocks, hipStrea
 
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st)
 {
-    const size_t lds = triplet3_lds_bytes(nb, a.ncfg, a.reps, a.kmax, fp64, weighted);
+    const size_t lds = triplet3_lds_bytes(a, nb, fp64, weighted);
     if (fp64) return weighted ? launch_t3<true, true>(a, lds, nblocks, st) : launch_t3<true, false>(a, lds, nblocks, st);
     return weighted ? launch_t3<false, true>(a, lds, nblocks, st) : launch_t3<false, false>(a, lds, nblocks, st);
 }
@@ -298,9 +348,9 @@ static int occ3(size_t lds)
     return nb > 0 ? nb : 1;
 }
 
-int triplet3_occupancy(int nb, int ncfg, int reps, int kmax, bool fp64, bool weighted)
+int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted)
 {
-    const size_t lds = triplet3_lds_bytes(nb, ncfg, reps, kmax, fp64, weighted);
+    const size_t lds = triplet3_lds_bytes(a, nb, fp64, weighted);
     if (fp64) return weighted ? occ3<true, true>(lds) : occ3<true, false>(lds);
     return weighted ? occ3<false, true>(lds) : occ3<false, false>(lds);
 }

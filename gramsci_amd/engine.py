@@ -107,6 +107,42 @@ class Engine:
                                                    len(ndata)))
         self.nreg = len(ndata)
 
+    def decompose(self, xyz, region, rmax, nreg=None):
+        """Domain decomposition on the GPU (gs_decompose): labels 1..nreg (default
+        max(region)) -> dict(off[nreg+1], idx, buf, box[nreg, 6]); region r's
+        members are idx[off[r]:off[r+1]] with buffer flags buf[...].  Single
+        precision, as the reference's tool."""
+        x = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        reg = np.ascontiguousarray(region, dtype=np.int32)
+        if nreg is None:
+            nreg = int(reg.max()) if len(reg) else 0
+        if nreg < 1:
+            raise ValueError("no region label >= 1")
+        off = np.zeros(nreg + 1, np.int64)
+        box = np.zeros((nreg, 6), np.float32)
+        fp = ctypes.c_float
+        L.check(self._lib.gs_decompose(self._ctx, L.ptr(x, fp), L.ptr(reg, ctypes.c_int32), len(x), fp(rmax), nreg,
+                                       L.ptr(off, ctypes.c_int64), None, None, 0, L.ptr(box, fp)))
+        idx = np.zeros(max(int(off[-1]), 1), np.int32)
+        buf = np.zeros(max(int(off[-1]), 1), np.int32)
+        L.check(self._lib.gs_decompose(self._ctx, L.ptr(x, fp), L.ptr(reg, ctypes.c_int32), len(x), fp(rmax), nreg,
+                                       L.ptr(off, ctypes.c_int64), L.ptr(idx, ctypes.c_int32),
+                                       L.ptr(buf, ctypes.c_int32), int(off[-1]), None))
+        return dict(off=off, idx=idx[:off[-1]], buf=buf[:off[-1]], box=box)
+
+    def set_catalogue_regions(self, xyz, w, region, rmax, nreg=None):
+        """Decomposes the catalogue on the GPU and sets the regions as a batched
+        catalogue: region r holds its members as the tool writes them to
+        "<r>.loadnodes" (single-precision coordinates and weights), every member
+        counted as data, with the tool's buffer flags.  pcf2_batched then gives
+        per region what the -cut run of that file computes with those flags
+        (its wsum class 0 is the un-normalised N2).  Returns the decomposition."""
+        d = self.decompose(xyz, region, rmax, nreg)
+        x = np.asarray(xyz, np.float32).reshape(-1, 3).astype(np.float64)[d["idx"]]
+        ws = None if w is None else np.asarray(w, np.float32).astype(np.float64)[d["idx"]]
+        self.set_catalogue_batched(x, ws, d["buf"], d["off"], np.diff(d["off"]))
+        return d
+
     def pcf2_batched(self, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
         p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
         R = self.nreg
@@ -157,16 +193,29 @@ class Engine:
         self.last_stats = st.as_dict()
         return res
 
-    def pcf3(self, rmin, rmax, nbins, log_bins=False, precision="fp32", cell_div=0):
-        p = self.params(rmin, rmax, nbins, 1, log_bins, precision, cell_div)
-        nc = n3cfg(nbins)
-        cnt = np.zeros((nc, 4), np.uint64)
-        wsum = np.zeros((nc, 4))
-        n2 = np.zeros(nc)
-        n3 = np.zeros(nc)
+    def pcf3(self, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
+        """3PCF over all configurations; nmu >= 2 is the RSD 3PCF.  Arrays are
+        [ncfg, 4] / [ncfg] for nmu == 1 and [ncfg, nmu, 4] / [ncfg, nmu] otherwise."""
+        return self._pcf3(self._lib.gs_count_3pcf, n3cfg(nbins), rmin, rmax, nbins, nmu, log_bins, precision,
+                          cell_div)
+
+    def pcf3_equi(self, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
+        """Equilateral 3PCF: arrays [nbins, nmu, 4] / [nbins, nmu] (the radial bin replaces the configuration)."""
+        r = self._pcf3(self._lib.gs_count_3pcf_equi, nbins, rmin, rmax, nbins, nmu, log_bins, precision, cell_div,
+                       squeeze=False)
+        return r
+
+    def _pcf3(self, fn, nc, rmin, rmax, nbins, nmu, log_bins, precision, cell_div, squeeze=True):
+        p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
+        cnt = np.zeros((nc, nmu, 4), np.uint64)
+        wsum = np.zeros((nc, nmu, 4))
+        n2 = np.zeros((nc, nmu))
+        n3 = np.zeros((nc, nmu))
         st = L.gs_stats()
-        L.check(self._lib.gs_count_3pcf(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(wsum),
-                                        L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
+        L.check(fn(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(wsum), L.ptr(n2), L.ptr(n3),
+                   ctypes.byref(st)))
+        if squeeze and nmu == 1:
+            cnt, wsum, n2, n3 = cnt[:, 0], wsum[:, 0], n2[:, 0], n3[:, 0]
         return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, zeta=_xi(n2, n3), stats=st.as_dict())
 
     def pcf4(self, rmin, rmax, nbins, log_bins=False, precision="fp64", cell_div=0):
@@ -180,4 +229,17 @@ class Engine:
         st = L.gs_stats()
         L.check(self._lib.gs_count_4pcf(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(n2),
                                         L.ptr(n3), L.ptr(dn), L.ptr(dr), ctypes.byref(st)))
+        return dict(cnt=cnt, n2=n2, n3=n3, disc_n=dn, disc_r=dr, eta=_xi(n2, n3), stats=st.as_dict())
+
+    def pcf4_tetra(self, rmin, rmax, nbins, log_bins=False, precision="fp64", cell_div=0):
+        """Regular-tetrahedron 4PCF: cnt[nbins, 16], n2/n3[nbins], disc_n/disc_r[6, nbins]."""
+        p = self.params(rmin, rmax, nbins, 1, log_bins, precision, cell_div)
+        cnt = np.zeros((nbins, 16), np.uint64)
+        n2 = np.zeros(nbins)
+        n3 = np.zeros(nbins)
+        dn = np.zeros((6, nbins))
+        dr = np.zeros((6, nbins))
+        st = L.gs_stats()
+        L.check(self._lib.gs_count_4pcf_tetra(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), L.ptr(n2),
+                                              L.ptr(n3), L.ptr(dn), L.ptr(dr), ctypes.byref(st)))
         return dict(cnt=cnt, n2=n2, n3=n3, disc_n=dn, disc_r=dr, eta=_xi(n2, n3), stats=st.as_dict())

@@ -1,11 +1,12 @@
 ! gramsci_host.F90 -- command-line host of the MI355X N-point correlation
 ! engine.  It keeps the reference program's flags and output-file layout
 ! (src/gramsci.F90: parseOptions :1166-1273, read_files_2 :1364-1396,
-! read_files :1342-1362, writers :382-397, :513-543, :829-858) and hands the
+! read_files :1342-1362, writers :382-397, :451-453, :513-543, :649-655,
+! :829-858) and hands the
 ! counting to libgramsci_hip.so through the C ABI in include/gramsci_hip.h.
 !
 !   gramsci -gal D [-ran R] -rmin a -rmax b -nbins n [-nmu m] [-log] [-out f]
-!           [-2pcf] [-3pcf] [-4pcf]  |  -cut file ...  |  -help
+!           [-2pcf] [-3pcf] [-equi] [-4pcf] [-tetra]  |  -cut file ...  |  -help
 !
 ! Extra option of this build: -fp64 counts with the plain FP64 kernels (the
 ! default is FP32 with an FP64 guard band; both give identical counts).
@@ -69,6 +70,23 @@ module gramsci_hip_api
        type(gs_stats), intent(out) :: st
      end function gs_count_4pcf
 
+     integer(c_int) function gs_count_3pcf_equi(ctx, p, cnt, wsum, n2, n3, st) bind(C, name='gs_count_3pcf_equi')
+       import :: c_int, c_ptr, gs_params, gs_stats
+       type(c_ptr), value :: ctx
+       type(gs_params), intent(in) :: p
+       type(c_ptr), value :: cnt, wsum, n2, n3
+       type(gs_stats), intent(out) :: st
+     end function gs_count_3pcf_equi
+
+     integer(c_int) function gs_count_4pcf_tetra(ctx, p, cnt, n2, n3, disc_n, disc_r, st) &
+          bind(C, name='gs_count_4pcf_tetra')
+       import :: c_int, c_ptr, gs_params, gs_stats
+       type(c_ptr), value :: ctx
+       type(gs_params), intent(in) :: p
+       type(c_ptr), value :: cnt, n2, n3, disc_n, disc_r
+       type(gs_stats), intent(out) :: st
+     end function gs_count_4pcf_tetra
+
      type(c_ptr) function gs_last_error() bind(C, name='gs_last_error')
        import :: c_ptr
      end function gs_last_error
@@ -122,15 +140,6 @@ program gramsci
   call defaults()
   call parse_options()
 
-  if (do_equi .or. do_tetra) then
-     print '(a)', 'gramsci: -equi and -tetra are not provided by this engine'
-     error stop 2
-  end if
-  if (do3 .and. nmu >= 2) then
-     print '(a)', 'gramsci: the anisotropic 3PCF (-3pcf with -nmu >= 2) is not provided by this engine'
-     error stop 2
-  end if
-
   if (cut_mode) then
      call read_cut()
   else
@@ -167,7 +176,9 @@ program gramsci
   ! each requested analysis writes the same output file, in the reference's order (gramsci.F90:193-214)
   if (do2) call run_2pcf()
   if (do3) call run_3pcf()
+  if (do_equi) call run_equi()
   if (do4) call run_4pcf()
+  if (do_tetra) call run_tetra()
 
   call check(gs_destroy(ctx), 'gs_destroy')
   print '(a)', ' Exit... stage left!'
@@ -194,7 +205,8 @@ contains
     print '(a)', ' gramsci: binned 2-, 3- and 4-point correlation counts on an MI355X'
     print '(a)', ' '
     print '(a)', '   gramsci -gal galaxy_file [-ran random_file] -rmin Rmin -rmax Rmax -nbins Nbins'
-    print '(a)', '           [-nmu Nmu] [-log] [-out out_file] [-fp64]   (one of -2pcf -3pcf -4pcf)'
+    print '(a)', '           [-nmu Nmu] [-log] [-out out_file] [-fp64]'
+    print '(a)', '           [-2pcf] [-3pcf] [-equi] [-4pcf] [-tetra]'
     print '(a)', '   gramsci -cut subsample_file ...    (columns x y z w buffer; weights not normalised)'
     print '(a)', ' '
     print '(a)', '   Input files hold x y z w per line.  Nmu >= 2 bins mu linearly in -1 < mu < 1.'
@@ -360,33 +372,109 @@ contains
   end subroutine run_2pcf
 
   subroutine run_3pcf()
-    integer(c_int64_t), allocatable, target :: cnt(:,:)
-    real(c_double), allocatable, target :: n2(:), n3(:)
+    integer(c_int64_t), allocatable, target :: cnt(:,:,:)
+    real(c_double), allocatable, target :: n2(:,:), n3(:,:)
     type(gs_stats) :: st
-    integer :: i1, i2, i3, cfg, ncfg
+    integer :: i1, i2, i3, l, cfg, ncfg
     ncfg = nbins*(nbins + 1)*(nbins + 2)/6
-    allocate(cnt(4, ncfg), n2(ncfg), n3(ncfg))
+    allocate(cnt(4, nmu, ncfg), n2(nmu, ncfg), n3(nmu, ncfg))
     call check(gs_count_3pcf(ctx, prm, c_loc(cnt), c_null_ptr, c_loc(n2), c_loc(n3), st), 'gs_count_3pcf')
     call report(st, '3PCF')
     if (cut_mode) then
        n2 = n2*cut_scale(3)
        n3 = 0.0d0
     end if
-    ! gramsci.F90:513-543, configurations in i <= j <= k order
+    ! gramsci.F90:513-543, configurations in i <= j <= k order; with nmu >= 2 one row per mu bin
     open(11, file=trim(file_out), status='unknown')
-    write(11, *) 'r1 min, r1 max, r2 min, r2 max, r3 min, r3 max, NNN, RRR, 3pcf (zeta)'
+    if (nmu >= 2) then
+       write(11, *) 'r1 min, r1 max, r2 min, r2 max, r3 min, r3 max, mu min, mu max, NNN, RRR, 3pcf (zeta)'
+    else
+       write(11, *) 'r1 min, r1 max, r2 min, r2 max, r3 min, r3 max, NNN, RRR, 3pcf (zeta)'
+    end if
     cfg = 0
     do i1 = 1, nbins
        do i2 = i1, nbins
           do i3 = i2, nbins
              cfg = cfg + 1
-             write(11, '(9(e14.7,1x))') edges(i1), edges(i1 + 1), edges(i2), edges(i2 + 1), edges(i3), &
-                  edges(i3 + 1), n2(cfg), n3(cfg), n2(cfg)/n3(cfg)
+             if (nmu >= 2) then
+                do l = 1, nmu
+                   write(11, '(11(e14.7,1x))') edges(i1), edges(i1 + 1), edges(i2), edges(i2 + 1), edges(i3), &
+                        edges(i3 + 1), ((float(l) - 1.)/mu_scale/2.), (float(l)/mu_scale/2.), n2(l, cfg), &
+                        n3(l, cfg), n2(l, cfg)/n3(l, cfg)
+                end do
+             else
+                write(11, '(9(e14.7,1x))') edges(i1), edges(i1 + 1), edges(i2), edges(i2 + 1), edges(i3), &
+                     edges(i3 + 1), n2(1, cfg), n3(1, cfg), n2(1, cfg)/n3(1, cfg)
+             end if
           end do
        end do
     end do
     close(11)
   end subroutine run_3pcf
+
+  ! -equi: triangles whose three sides share one bin (gramsci.F90:402-461); no header line
+  subroutine run_equi()
+    integer(c_int64_t), allocatable, target :: cnt(:,:,:)
+    real(c_double), allocatable, target :: n2(:,:), n3(:,:)
+    type(gs_stats) :: st
+    integer :: l, k
+    allocate(cnt(4, nmu, nbins), n2(nmu, nbins), n3(nmu, nbins))
+    call check(gs_count_3pcf_equi(ctx, prm, c_loc(cnt), c_null_ptr, c_loc(n2), c_loc(n3), st), &
+         'gs_count_3pcf_equi')
+    call report(st, 'equilateral 3PCF')
+    if (cut_mode) then
+       n2 = n2*cut_scale(3)
+       n3 = 0.0d0
+    end if
+    open(11, file=trim(file_out), status='unknown')
+    do l = 1, nbins
+       do k = 1, nmu
+          write(11, '(8(e14.7,1x))') edges(l), edges(l + 1), ((float(k) - 1.)/mu_scale/2.), &
+               (float(k)/mu_scale/2.), n2(k, l), n3(k, l), n2(k, l)/n3(k, l)
+       end do
+    end do
+    close(11)
+  end subroutine run_equi
+
+  ! -tetra: regular tetrahedra (gramsci.F90:548-660); no header line.  The
+  ! counts have no mu dependence: rows k >= 2 print zero counts, as the
+  ! reference's N2(l,k,3) = 0 for k >= 2.
+  subroutine run_tetra()
+    integer(c_int64_t), allocatable, target :: cnt(:,:)
+    real(c_double), allocatable, target :: n2(:), n3(:), dn(:,:), dr(:,:)
+    real(c_double) :: a2, a3
+    type(gs_params) :: p1
+    type(gs_stats) :: st
+    integer :: l, k
+    allocate(cnt(16, nbins), n2(nbins), n3(nbins), dn(nbins, 6), dr(nbins, 6))
+    p1 = prm
+    p1%nmu = 1
+    call check(gs_count_4pcf_tetra(ctx, p1, c_loc(cnt), c_loc(n2), c_loc(n3), c_loc(dn), c_loc(dr), st), &
+         'gs_count_4pcf_tetra')
+    call report(st, 'tetrahedral 4PCF')
+    if (cut_mode) then
+       n2 = n2*cut_scale(4)
+       n3 = 0.0d0
+       dn = dn*cut_scale(3)
+       dr = 0.0d0
+    end if
+    open(11, file=trim(file_out), status='unknown')
+    do l = 1, nbins
+       do k = 1, nmu
+          a2 = 0.0d0
+          a3 = 0.0d0
+          if (k == 1) then
+             a2 = n2(l)
+             a3 = n3(l)
+          end if
+          write(11, '(9(e14.7,1x))') edges(l), edges(l + 1), ((float(k) - 1.)/mu_scale/2.), &
+               (float(k)/mu_scale/2.), a2, a3, a2/a3, &
+               (dn(l, 1)/dr(l, 1))*(dn(l, 5)/dr(l, 5)) + (dn(l, 3)/dr(l, 3))*(dn(l, 4)/dr(l, 4)) &
+               + (dn(l, 2)/dr(l, 2))*(dn(l, 6)/dr(l, 6))
+       end do
+    end do
+    close(11)
+  end subroutine run_tetra
 
   subroutine run_4pcf()
     integer(c_int64_t), allocatable, target :: cnt(:,:,:,:)

@@ -105,12 +105,22 @@ int gs_count_2pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
                   double* n2, double* n3, gs_stats* st);
 
 /* 3PCF, all configurations: create_graph + query_graph_3pcf_all (gramsci.F90:463-545).
- * nmu must be 1.  ncfg = C(nbins+2,3), in the reference's i<=j<=k order.
- *   cnt [ncfg][4]   apex-ordered triplet counts by number of randoms among (apex, k1, k2)
- *   wsum[ncfg][4]   sum of raw-weight triple products per class (may be NULL)
- *   n2, n3 [ncfg]   N2(:,1,3), N3(:,1,3) */
+ * ncfg = C(nbins+2,3), in the reference's i<=j<=k order.  nmu >= 2 is the
+ * reference's RSD 3PCF: each triplet also goes to the mu bin
+ * find_normal(mu(k1), mu(k2)) (:495-500, :1611-1625).
+ *   cnt [ncfg][nmu][4]   apex-ordered triplet counts by number of randoms among (apex, k1, k2)
+ *   wsum[ncfg][nmu][4]   sum of raw-weight triple products per class (may be NULL)
+ *   n2, n3 [ncfg][nmu]   N2(:,:,3), N3(:,:,3) */
 int gs_count_3pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
                   double* n2, double* n3, gs_stats* st);
+
+/* Equilateral 3PCF: create_graph + query_graph_equilateral_triangle
+ * (gramsci.F90:402-461, option -equi): triangles whose three sides share one
+ * radial bin b.  Same outputs as gs_count_3pcf with the configuration replaced
+ * by b: cnt/wsum [nbins][nmu][4], n2/n3 [nbins][nmu].  With nmu >= 2 the
+ * reference accumulates no N3 (:429-431): n3 is 0. */
+int gs_count_3pcf_equi(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
+                       double* n2, double* n3, gs_stats* st);
 
 /* 4PCF: create_graph + query_4pcf_all2 (gramsci.F90:662-861).  nmu must be 1.
  * Cells are (cfg, ind2, ind6) with ncfg = C(nbins+3,4): ncell = ncfg*nbins*nbins.
@@ -119,6 +129,15 @@ int gs_count_3pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
  *   disc_n, disc_r [6][ncell] discN(bin,ind2,ind6,1..6), discR(...) (may be NULL) */
 int gs_count_4pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* n2, double* n3,
                   double* disc_n, double* disc_r, gs_stats* st);
+
+/* Regular-tetrahedron 4PCF: create_graph + query_graph_tetrahedron
+ * (gramsci.F90:548-660, option -tetra): apex and neighbours k1 < k2 < k3 (by
+ * original index) whose six sides share one radial bin b.  nmu must be 1.
+ *   cnt [nbins][16]           counts by random pattern (bit0 apex, bit1 k1, bit2 k2, bit3 k3)
+ *   n2, n3 [nbins]            N2(b,1,3), N3(b,1,3)
+ *   disc_n, disc_r [6][nbins] discN(b,1,1,1..6), discR(b,1,1,1..6) (may be NULL) */
+int gs_count_4pcf_tetra(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* n2, double* n3,
+                        double* disc_n, double* disc_r, gs_stats* st);
 
 /* Batched independent regions (domain-decomposed sub-volumes with their
  * buffer shells, jackknife regions): region r holds points [off[r], off[r+1])
@@ -136,6 +155,23 @@ int gs_set_catalogue_batched(gs_ctx* ctx, const double* xyz, const double* w, co
  * the captured pipeline for 2..6 cells per rmax (cached per parameter set). */
 int gs_count_2pcf_batched(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
                           double* n2, double* n3, gs_stats* st);
+
+/* Domain decomposition on the GPU, replacing the reference's offline tool
+ * (domain_decomposition.f90:35-74).  region[i] in 1..nreg labels point i (other
+ * labels belong to no region).  For each label r: box = bounding box of the
+ * points labelled r; members = every point strictly inside the box grown by
+ * rmax on every side; buffer flag 0 for a member inside the closed box, 1
+ * otherwise.  As in the tool, coordinates, rmax and the boxes are single
+ * precision.  A label with no point has box (+FLT_MAX, -FLT_MAX) and no member.
+ *   off[nreg+1]  member offsets (always written)
+ *   idx, buf     [off[nreg]] member indices (input order within a region) and
+ *                buffer flags, region-major; both NULL to query the sizes only,
+ *                else cap >= off[nreg] (GS_ELIMIT otherwise, off still written)
+ *   box[nreg][6] xmin, ymin, zmin, xmax, ymax, zmax (may be NULL)
+ * The members of region r with their flags are the catalogue the tool writes to
+ * "r.loadnodes" and, fed to gs_set_catalogue_batched, one region of a batch. */
+int gs_decompose(gs_ctx* ctx, const float* xyz, const int32_t* region, int64_t n, float rmax, int nreg,
+                 int64_t* off, int32_t* idx, int32_t* buf, int64_t cap, float* box);
 
 /* Adjoint of the weighted 2PCF: given dl_dn2 = dL/dN2 and dl_dn3 = dL/dN3
  * [nbins][nmu] (host memory) for the catalogue of the context, writes

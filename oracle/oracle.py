@@ -269,3 +269,37 @@ def reference_table(xyz_d, w_d, xyz_r, w_r, p: Params, mode="-2pcf", threads=Non
         with open(os.path.join(td, "o.txt")) as f:
             text = f.read()
     return text
+
+
+def decompose(xyz, region, rmax, nreg=None):
+    """The reference's domain decomposition tool (domain_decomposition.f90:35-74)
+    in single precision, as the tool's default reals: for labels r = 1..nreg,
+    box = min/max of the points labelled r (:38-45), grown by rmax (:48-53);
+    members = points strictly inside the grown box (:62-64), in input order;
+    buffer = 0 inside the closed box, else 1 (:67-70).  An empty label has box
+    (+huge, -huge) and no member.  Returns (off[nreg+1], idx, buf, box[nreg,6])
+    with box rows xmin, ymin, zmin, xmax, ymax, zmax."""
+    x = np.asarray(xyz, np.float32)
+    region = np.asarray(region)
+    if nreg is None:
+        nreg = int(region.max()) if len(region) else 0
+    r32 = np.float32(rmax)
+    off, idx, buf, boxes = [0], [], [], []
+    big = np.finfo(np.float32).max
+    for r in range(1, nreg + 1):
+        m = region == r
+        if m.any():
+            lo, hi = x[m].min(0), x[m].max(0)
+        else:
+            lo, hi = np.full(3, big, np.float32), np.full(3, -big, np.float32)
+        lo2, hi2 = lo - r32, hi + r32            # float32 arithmetic
+        inside = np.all((x > lo2) & (x < hi2), axis=1)
+        core = np.all((x >= lo) & (x <= hi), axis=1)
+        k = np.nonzero(inside)[0]
+        idx.append(k)
+        buf.append(np.where(core[k], 0, 1))
+        off.append(off[-1] + len(k))
+        boxes.append(np.concatenate([lo, hi]))
+    cat = (lambda a, t: np.concatenate(a).astype(t) if a else np.zeros(0, t))
+    return (np.array(off, np.int64), cat(idx, np.int32), cat(buf, np.int32),
+            np.array(boxes, np.float32).reshape(-1, 6))

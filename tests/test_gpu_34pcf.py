@@ -49,7 +49,7 @@ def check3(eng, xyz, w, nd, p, buf=None, precisions=("fp64", "fp32")):
     outs = []
     for prec in precisions:
         eng.set_catalogue(xyz, w, nd, buf)
-        r = eng.pcf3(p.rmin, p.rmax, p.nbins, p.log_bins, prec)
+        r = eng.pcf3(p.rmin, p.rmax, p.nbins, nmu=p.nmu, log_bins=p.log_bins, precision=prec)
         print(f"  3pcf {prec}: units={r['stats']['n_units']} binned={r['stats']['n_binned']} "
               f"recheck={r['stats']['n_recheck']}", flush=True)
         assert np.array_equal(r["cnt"], ref["cnt"]), (prec, np.argwhere(r["cnt"] != ref["cnt"])[:5])
@@ -79,9 +79,10 @@ def test_3pcf_golden(eng, name):
     print(f"case {name}", flush=True)
     ref, outs = check3(eng, xyz, w, nd, p)
     rows = parse_rows(read_text(name, "DR" if "DR" in c["runs"] else "D"))
+    col = 8 if p.nmu > 1 else 6      # RSD rows carry two mu columns (gramsci.F90:531-532)
     for r in outs:
-        assert close_e147(rows[:, 6], r["n2"]).all()
-        assert close_e147(rows[:, 7], r["n3"]).all()
+        assert close_e147(rows[:, col], r["n2"].reshape(-1)).all()
+        assert close_e147(rows[:, col + 1], r["n3"].reshape(-1)).all()
 
 
 @pytest.mark.parametrize("name", [n for n, c in MAN.items() if c["mode"] == "-4pcf"])
@@ -102,7 +103,7 @@ def test_3pcf_exact_counts_vs_reference_runs(eng):
     d, r, _, _ = load_input(name)
     for run, cat in (("D", d), ("R", r), ("T", np.concatenate([d, r]))):
         eng.set_catalogue(cat, None, len(cat))
-        out = eng.pcf3(p.rmin, p.rmax, p.nbins, False, "fp32")
+        out = eng.pcf3(p.rmin, p.rmax, p.nbins, precision="fp32")
         rows = parse_rows(read_text(name, run))
         got = np.rint(rows[:, 6] * float(len(cat)) ** 3).astype(np.int64)
         assert np.array_equal(out["cnt"][:, 0].astype(np.int64), got), run
@@ -160,18 +161,17 @@ def test_4pcf_lattice_and_log(eng):
     check4(eng, d, None, len(d), O.Params(1.0, 8.0, 4, 1, True))
 
 
-def test_3pcf_rejects_rsd_and_large_neighbour_sets(eng):
+def test_4pcf_rejects_mu_bins_and_3pcf_large_neighbour_sets(eng):
     from gramsci_amd._lib import GsError
     d = C.uniform(2000, 50.0, 1)
     eng.set_catalogue(d, None, len(d))
-    with pytest.raises(GsError):
-        eng.pcf2(1.0, 5.0, 3, 1, False, "fp32")  # fine
-        raise GsError(-1, "sentinel")
     p = Engine.params(1.0, 5.0, 3, nmu=4)
     import ctypes
     from gramsci_amd import _lib as L
-    cnt = np.zeros((10, 4), np.uint64)
-    rc = L.lib().gs_count_3pcf(eng._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64), None, None, None, None)
+    # the 4PCF has no mu binning; rejected before any output is written
+    rc = L.lib().gs_count_4pcf(eng._ctx, ctypes.byref(p), None, None, None, None, None, None)
+    assert rc == L.GS_EINVAL
+    rc = L.lib().gs_count_4pcf_tetra(eng._ctx, ctypes.byref(p), None, None, None, None, None, None)
     assert rc == L.GS_EINVAL
     # every point within rmax of every other: more neighbours than the LDS list holds
     dense = C.uniform(9000, 10.0, 2)
@@ -179,3 +179,26 @@ def test_3pcf_rejects_rsd_and_large_neighbour_sets(eng):
     with pytest.raises(GsError) as e:
         eng.pcf3(0.1, 30.0, 3)
     assert e.value.code == L.GS_ELIMIT
+
+
+def test_3pcf_rsd_exact_counts_vs_reference_runs(eng):
+    name = "p3_rsd"
+    p = O.Params(**MAN[name]["params"])
+    d, r, _, _ = load_input(name)
+    for run, cat in (("D", d), ("R", r), ("T", np.concatenate([d, r]))):
+        eng.set_catalogue(cat, None, len(cat))
+        out = eng.pcf3(p.rmin, p.rmax, p.nbins, nmu=p.nmu, precision="fp32")
+        rows = parse_rows(read_text(name, run))
+        got = np.rint(rows[:, 8] * float(len(cat)) ** 3).astype(np.int64)
+        assert np.array_equal(out["cnt"][..., 0].reshape(-1).astype(np.int64), got), run
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_3pcf_rsd_global_histogram(eng, weighted):
+    # 12 bins x 10 mu bins: the histogram does not fit in LDS and goes to global atomics
+    xyz = C.clustered(6000, 120.0, 77, members=10, sigma=2.0)
+    nd = 3000
+    w = C.lognormal_weights(len(xyz), 0.3, 78) if weighted else None
+    p = O.Params(3.0, 15.0, 12, 10)
+    print(f"case rsd global hist weighted={weighted}", flush=True)
+    check3(eng, xyz, w, nd, p)

@@ -39,9 +39,11 @@ def test_unknown_option_stops(tmp_path):
 
 
 def test_options_match_on_six_characters(tmp_path):
-    # the reference reads options into a character(len=6): "-nbinsXYZ" is "-nbins"
-    r = run(["-gal", "a", "-rmin", "1", "-rmax", "2", "-nbinsXYZ", "3", "-equi"], tmp_path)
-    assert "not provided by this engine" in r.stdout
+    # the reference reads options into a character(len=6): "-nbinsXYZ" is "-nbins",
+    # so parsing goes on to the (missing) input file instead of stopping on the option
+    r = run(["-gal", "missing.txt", "-rmin", "1", "-rmax", "2", "-nbinsXYZ", "3", "-equi"], tmp_path)
+    assert "unknown option" not in r.stdout
+    assert r.returncode != 0
 
 
 def _args(c, gal, ran, out):
@@ -61,7 +63,9 @@ def _compare(name, run_name, text):
     ref = read_text(name, run_name)
     rl, gl = ref.splitlines(), text.splitlines()
     assert len(rl) == len(gl)
-    ncols_fixed = {"-2pcf": 4, "-3pcf": 6, "-4pcf": 8}[MAN[name]["mode"]]
+    ncols_fixed = {"-2pcf": 4, "-3pcf": 6, "-4pcf": 8, "-equi": 4, "-tetra": 4}[MAN[name]["mode"]]
+    if MAN[name]["mode"] == "-3pcf" and MAN[name]["params"]["nmu"] > 1:
+        ncols_fixed = 8            # RSD rows add mu min, mu max
     identical = 0
     for a, b in zip(rl, gl):
         ta, tb = a.split(), b.split()

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgramsci_hip.so")
+# GS_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GS_LIBRARY") or os.path.join(HERE, "libgramsci_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "gramsci_hip.h")
 
 GS_OK, GS_EINVAL, GS_ENOMEM, GS_EHIP, GS_ENODEV, GS_ELIMIT = 0, -1, -2, -3, -4, -5
@@ -90,6 +91,7 @@ def lib():
     L.gs_decompose.argtypes = [vp, P(ctypes.c_float), i32p, ctypes.c_int64, ctypes.c_float, ctypes.c_int, i64p, i32p,
                                i32p, ctypes.c_int64, P(ctypes.c_float)]
     L.gs_count_2pcf_batched.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
+    L.gs_count_3pcf_batched.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
     L.gs_adjoint_2pcf_device.argtypes = [vp, pp, dp, dp, vp, sp]
     for name in exported_names():
         getattr(L, name)  # every declared symbol must resolve

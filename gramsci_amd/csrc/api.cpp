@@ -226,6 +226,37 @@ int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc
         B.ua = (float)inv;
         B.ub = (float)(-p.rmin * inv);
     }
+    // radial guard of the FP32 pair path in bin units.  The kernel's bin
+    // coordinate is u = fl(fl(log2 s32)*ua + ub) (log bins) or
+    // u = fl(fl(s32*rsq(s32))*ua + ub) (linear bins), against U(s) of the exact
+    // s, whose integer crossings are the thresholds t_k (to ~1e-13).  With
+    // |s32 - s| <= g(s) (s_guard), the bound is |dU/ds| g(s) + the log2 /
+    // square-root error (taken as 2^-16 absolute and 2^-20 relative, far above
+    // the hardware's ~1 ulp) + the fma rounding + 1e-9, times 1.25:
+    //   log:    dU/ds = ua/(s ln2)
+    //   linear: dU/ds = ua/(2 sqrt(s))
+    {
+        const double r3 = 2.0 * std::sqrt(3.0);
+        const double fma_err = std::ldexp((double)nb + 2.0, -23) + 1e-9;
+        double d0, d1, d2, d3;
+        if (p.log_bins) {
+            const double k = 2.0 * (double)B.ua / std::log(2.0);
+            d0 = k * std::ldexp(1.0, -22) + (double)B.ua * std::ldexp(1.0, -16) + fma_err;
+            d1 = k * r3 * ew;
+            d2 = k * 3.0 * ew * ew;
+            d3 = 0.0;
+        } else {
+            const double ua = B.ua;
+            d0 = ua * r3 * ew + ua * std::ldexp(1.0, -16) + fma_err;
+            d1 = ua * 3.0 * ew * ew;
+            d2 = 0.0;
+            d3 = ua * (std::ldexp(1.0, -22) + std::ldexp(1.0, -20));
+        }
+        B.ud0 = f_up(1.25 * d0);
+        B.ud1 = f_up(1.25 * d1);
+        B.ud2 = f_up(1.25 * d2);
+        B.ud3 = f_up(1.25 * d3);
+    }
     // mu guard: per-component errors of sep (ew) and of the absolute midpoint (em)
     const double em = std::ldexp(2.0 * xmax + 3.0 * rrel, -24) * 1.01;
     B.mu_ew = (float)(2.0 * std::sqrt(3.0) * ew * 4.0);
@@ -613,10 +644,14 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     BinDesc B;
     if ((rc = fill_bins(*p, (mst + 1) * h, xmax, c->wmax, B))) return rc;
 
-    // LDS histogram replicas: as many as fit in 32 KB of counters, at most 32
-    int reps = 1;
-    while (reps < 32 && (size_t)nbt * reps * 2 * 4 <= 32 * 1024) reps *= 2;
-    while (reps > 1 && pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted) > 64 * 1024) reps /= 2;
+    // LDS histogram replicas (spreading same-bin atomics): the most replicas, at
+    // most 32, that keep the largest number of resident work-groups per CU
+    int reps = 1, best_occ = 0;
+    for (int r = 1; r <= 32; r *= 2) {
+        if (pair2_lds_bytes(nb, rsd ? nm : 1, r, fp64, weighted) > 64 * 1024) break;
+        const int occ = pair2_occupancy(nb, nm, r, fp64, rsd, weighted, p->log_bins != 0);
+        if (occ >= best_occ) { best_occ = occ; reps = r; }
+    }
     if (const char* ev = getenv("GS_PAIR_REPS")) {   // tuning override (power of two)
         int v = atoi(ev);
         if (v >= 1 && v <= 32 && (v & (v - 1)) == 0) reps = v;

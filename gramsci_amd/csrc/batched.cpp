@@ -41,10 +41,11 @@ void batch_release(GsBatch& b)
     b.valid = false;
 }
 
-static std::string plan_key(const gs_params& p)
+static std::string plan_key(const gs_params& p, int kind)
 {
     char k[160];
-    snprintf(k, sizeof(k), "%.17g/%.17g/%d/%d/%d/%d", p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, p.precision);
+    snprintf(k, sizeof(k), "%d:%.17g/%.17g/%d/%d/%d/%d", kind, p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins,
+             p.precision);
     return k;
 }
 
@@ -54,13 +55,14 @@ static bool same_params(const gs_params& a, const gs_params& b)
            a.log_bins == b.log_bins && a.precision == b.precision;
 }
 
-// Builds (or finds) the captured pipeline for parameters p at m cells per rmax.
-static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
+// Builds (or finds) the captured pipeline for parameters p at m cells per
+// rmax, ending in the pair kernel (kind 2) or the triplet kernel (kind 3).
+static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Plan** out)
 {
     CtxView v = view(c);
     GsBatch& B = *v.batch;
     for (auto& pl : B.plans)
-        if (pl.m == m && same_params(pl.p, p)) {
+        if (pl.m == m && pl.kind == kind && same_params(pl.p, p)) {
             *out = &pl;
             return GS_OK;
         }
@@ -95,12 +97,26 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     }
     row_base[nreg] = (int32_t)rows_tot;
     const int mst = (int)std::ceil(p.rmax / h - 1e-12);
-    std::vector<Row> rows = half_stencil(h, p.rmax, mst);
+    std::vector<Row> rows = kind == 3 ? full_stencil(h, p.rmax, mst) : half_stencil(h, p.rmax, mst);
+    if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
     BinDesc bins;
     int rc;
-    if ((rc = fill_bins(p, (mst + 1) * h, B.xmax, B.wmax, bins))) return rc;
+    if ((rc = fill_bins(p, kind == 3 ? 1.5 * p.rmax : (mst + 1) * h, B.xmax, B.wmax, bins))) return rc;
 
-    const size_t nout = (size_t)nreg * 3 * nbt + 4;
+    // triplet kernel: configuration and find_normal tables, LDS plan
+    std::vector<uint16_t> tab3;
+    std::vector<uint8_t> ntab;
+    TripArgs ta{};
+    int nh = 0;
+    if (kind == 3) {
+        tab3 = table3(nb);
+        if (nm > 1) ntab = normal_table(nm);
+        ta.ncfg = nb * (nb + 1) * (nb + 2) / 6;
+        ta.nm = nm;
+        if ((rc = plan_triplet_lds(ta, nb, fp64, weighted))) return rc;
+        nh = ta.ncfg * nm * 4;
+    }
+    const size_t nout = kind == 3 ? (size_t)nreg * 3 * nh + 8 : (size_t)nreg * 3 * nbt + 4;
     const int64_t scan_n = std::max<int64_t>(cells + 1, rows_tot + 1);
     // captured graphs hold raw pointers to the shared work buffers: if any of
     // them has to grow, every existing plan is destroyed before reallocating
@@ -118,7 +134,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     }
     if ((rc = B.cell_of.reserve(n)) || (rc = B.count.reserve(cells + 1)) || (rc = B.cell_start.reserve(cells + 1)) ||
         (rc = B.nch.reserve(rows_tot + 1)) || (rc = B.row_chunk.reserve(rows_tot + 1)) ||
-        (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(8)) ||
+        (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(16)) ||
         (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n)) || (rc = B.flags.reserve(n)) ||
         (rc = B.out.reserve(nout)))
         return rc;
@@ -126,7 +142,9 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     const size_t reg_bytes = ((nreg * sizeof(RegionDesc) + 15) / 16) * 16;
     const size_t bin_off = reg_bytes, rows_off = bin_off + ((sizeof(BinDesc) + 15) / 16) * 16;
     const size_t rb_off = rows_off + ((rows.size() * sizeof(Row) + 15) / 16) * 16;
-    const size_t blob = rb_off + (nreg + 1) * sizeof(int32_t) + 16;
+    const size_t tab_off = rb_off + (((nreg + 1) * sizeof(int32_t) + 15) / 16) * 16;
+    const size_t ntab_off = tab_off + ((tab3.size() * 2 + 15) / 16) * 16;
+    const size_t blob = ntab_off + ntab.size() + 16;
     // each plan owns its blob so that graphs of several parameter sets can coexist
     B.plans.emplace_back();
     GsBatch::Plan& P = B.plans.back();
@@ -140,6 +158,10 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     if (e == hipSuccess)
         e = hipMemcpyAsync(pblob->ptr + rb_off, row_base.data(), (nreg + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
                            v.stream);
+    if (e == hipSuccess && !tab3.empty())
+        e = hipMemcpyAsync(pblob->ptr + tab_off, tab3.data(), tab3.size() * 2, hipMemcpyHostToDevice, v.stream);
+    if (e == hipSuccess && !ntab.empty())
+        e = hipMemcpyAsync(pblob->ptr + ntab_off, ntab.data(), ntab.size(), hipMemcpyHostToDevice, v.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
     if (e != hipSuccess) { pblob->release(); B.plans.pop_back(); return hip_fail(e, "batch upload"); }
     const RegionDesc* d_reg = (const RegionDesc*)pblob->ptr;
@@ -166,17 +188,42 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     a.row_base = (const int32_t*)(pblob->ptr + rb_off);
     a.n_chunks_dev = B.row_chunk.ptr + rows_tot;
     a.nreg = nreg;
-    if (pair2_lds_bytes(nb, rsd ? nm : 1, 1, fp64, weighted, true) > 64 * 1024) {
+    if (kind == 2 && pair2_lds_bytes(nb, rsd ? nm : 1, 1, fp64, weighted, true) > 64 * 1024) {
         pblob->release(); B.plans.pop_back();
         return fail(GS_ELIMIT, "per-wave histograms of %d bins do not fit in LDS", nbt);
     }
-    const int occ = pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true);
-    const int nblocks = v.n_cu * occ;
+    if (kind == 3) {
+        ta.pw = B.pw.ptr;
+        ta.flags = B.flags.ptr;
+        ta.orig = B.orig.ptr;
+        ta.cell_start = B.cell_start.ptr;
+        ta.rows = (const Row*)(pblob->ptr + rows_off);
+        ta.bins = (const BinDesc*)(pblob->ptr + bin_off);
+        ta.cfg_tab = (const uint16_t*)(pblob->ptr + tab_off);
+        ta.ntab = ntab.empty() ? nullptr : (const uint8_t*)(pblob->ptr + ntab_off);
+        ta.queue = B.queue.ptr;
+        ta.err = B.queue.ptr + 8;
+        ta.out_cnt = B.out.ptr;
+        ta.out_w = B.out.ptr + (size_t)nreg * nh;
+        ta.out_stats = B.out.ptr + (size_t)nreg * 3 * nh;
+        ta.g.h = h;
+        ta.g.h32 = (float)h;
+        ta.g.m = mst;
+        ta.nrows = (int)rows.size();
+        ta.n_items = (int)n;
+        ta.equi = 0;
+        ta.nreg = nreg;
+        ta.regions = d_reg;
+    }
+    const int occ = kind == 3 ? triplet3_occupancy(ta, nb, fp64, weighted)
+                              : pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true);
+    const int nblocks = kind == 3 ? std::max(1, (int)std::min<int64_t>((int64_t)v.n_cu * occ, std::max<int64_t>(n, 1)))
+                                  : v.n_cu * occ;
 
-    // capture: clear, count sort into cells, scans, scatter, row chunks, pair kernel
+    // capture: clear, count sort into cells, scans, scatter, row chunks, pair or triplet kernel
     e = hipStreamBeginCapture(v.stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
-    if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 8, 0, v.stream);
+    if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 16, 0, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.out.ptr, (int64_t)nout * 2, 0, v.stream);
     if (e == hipSuccess)
         e = launch_cell_count_b(v.xyz, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
@@ -185,10 +232,14 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     if (e == hipSuccess)
         e = launch_scatter_b(v.xyz, v.w, v.buf, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.cell_start.ptr,
                              B.count.ptr, B.pw.ptr, B.pl.ptr, B.flags.ptr, B.orig.ptr, v.stream);
-    if (e == hipSuccess)
-        e = launch_row_chunks_b(B.cell_start.ptr, d_reg, a.row_base, nreg, (int)rows_tot, B.nch.ptr, v.stream);
-    if (e == hipSuccess) e = exclusive_scan(B.nch.ptr, B.row_chunk.ptr, rows_tot + 1, B.scan_tmp.ptr, v.stream);
-    if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p.log_bins != 0, nblocks, v.stream);
+    if (kind == 2) {
+        if (e == hipSuccess)
+            e = launch_row_chunks_b(B.cell_start.ptr, d_reg, a.row_base, nreg, (int)rows_tot, B.nch.ptr, v.stream);
+        if (e == hipSuccess) e = exclusive_scan(B.nch.ptr, B.row_chunk.ptr, rows_tot + 1, B.scan_tmp.ptr, v.stream);
+        if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p.log_bins != 0, nblocks, v.stream);
+    } else if (e == hipSuccess && n > 0) {
+        e = launch_triplet3(ta, nb, fp64, weighted, nblocks, v.stream);
+    }
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(v.stream, &graph);
     if (e == hipSuccess) e = e2;
@@ -204,8 +255,11 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, GsBatch::Plan** out)
     P.graph = graph;
     P.exec = exec;
     P.nbt = nbt;
-    P.fx_shift = bins.fx_shift2;
+    P.fx_shift = kind == 3 ? bins.fx_shift3 : bins.fx_shift2;
     P.mst = mst;
+    P.kind = kind;
+    P.nh = nh;
+    if (kind == 3) P.kmax = ta.kmax;
     *out = &P;
     return GS_OK;
 }
@@ -293,7 +347,7 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
 
     int m = p->cell_div;
     if (m == 0) {
-        const std::string key = plan_key(*p);
+        const std::string key = plan_key(*p, 2);
         for (auto& kv : B.tuned)
             if (kv.first == key) m = kv.second;
         if (m == 0) {
@@ -301,7 +355,7 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
             float best = 1e30f;
             for (int cand = 2; cand <= 6; ++cand) {
                 GsBatch::Plan* P;
-                if ((rc = get_plan(c, *p, cand, &P))) return rc;
+                if ((rc = get_plan(c, *p, cand, 2, &P))) return rc;
                 float t = 0;
                 if ((rc = run_plan(c, *P, nullptr)) || (rc = run_plan(c, *P, &t))) return rc;
                 if (t < best) { best = t; m = cand; }
@@ -310,7 +364,7 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         }
     }
     GsBatch::Plan* P;
-    if ((rc = get_plan(c, *p, m, &P))) return rc;
+    if ((rc = get_plan(c, *p, m, 2, &P))) return rc;
     float ms = 0;
     if ((rc = run_plan(c, *P, &ms))) return rc;
     const size_t nout = (size_t)nreg * 3 * nbt + 4;
@@ -349,6 +403,76 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         st->n_candidates = s[0];
         st->n_recheck = s[1];
         st->n_overflow = s[2];
+        st->cell_div = P->mst;
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2,
+                                     double* n3, gs_stats* st)
+{
+    if (!c) return fail(GS_EINVAL, "NULL context");
+    int rc = check_params(p, 3);
+    if (rc) return rc;
+    if ((rc = ctx_set_device(c))) return rc;
+    CtxView v = view(c);
+    GsBatch& B = *v.batch;
+    if (!B.valid) return fail(GS_EINVAL, "no batched catalogue (gs_set_catalogue_batched)");
+    const int nreg = B.nreg, nb = p->nbins, nm = p->nmu;
+    const int ncfg = nb * (nb + 1) * (nb + 2) / 6;
+    const bool weighted = v.has_w;
+
+    int m = p->cell_div;
+    if (m == 0) {
+        const std::string key = plan_key(*p, 3);
+        for (auto& kv : B.tuned)
+            if (kv.first == key) m = kv.second;
+        if (m == 0) {
+            float best = 1e30f;
+            for (int cand = 1; cand <= 3; ++cand) {
+                GsBatch::Plan* P;
+                if ((rc = get_plan(c, *p, cand, 3, &P))) return rc;
+                float t = 0;
+                if ((rc = run_plan(c, *P, nullptr)) || (rc = run_plan(c, *P, &t))) return rc;
+                if (t < best) { best = t; m = cand; }
+            }
+            B.tuned.emplace_back(key, m);
+        }
+    }
+    GsBatch::Plan* P;
+    if ((rc = get_plan(c, *p, m, 3, &P))) return rc;
+    float ms = 0;
+    if ((rc = run_plan(c, *P, &ms))) return rc;
+    const int nh = P->nh;
+    const size_t nout = (size_t)nreg * 3 * nh + 8;
+    std::vector<unsigned long long> h(nout);
+    int32_t err = 0;
+    hipError_t e = hipMemcpy(h.data(), B.out.ptr, nout * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&err, B.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "batched result copy");
+    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", P->kmax);
+    std::vector<long double> S(nh);
+    for (int r = 0; r < nreg; ++r) {
+        const unsigned long long* hc = h.data() + (size_t)r * nh;
+        const unsigned long long* hw = h.data() + (size_t)nreg * nh + (size_t)r * 2 * nh;
+        for (int k = 0; k < nh; ++k) {
+            if (cnt) cnt[(size_t)r * nh + k] = hc[k];
+            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift) : (long double)hc[k];
+            if (wsum) wsum[(size_t)r * nh + k] = (double)S[k];
+        }
+        combine3(S.data(), ncfg * nm, B.wd[r], B.wr[r], true, n2 ? n2 + (size_t)r * ncfg * nm : nullptr,
+                 n3 ? n3 + (size_t)r * ncfg * nm : nullptr);
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->ms_count = ms;
+        st->ms_kernel_avg = ms;
+        const unsigned long long* s = h.data() + (size_t)nreg * 3 * nh;
+        st->n_candidates = s[0];
+        st->n_recheck = s[1];
+        st->n_overflow = s[2];
+        st->n_units = s[3];
+        st->n_binned = s[4];
         st->cell_div = P->mst;
     }
     return GS_OK;

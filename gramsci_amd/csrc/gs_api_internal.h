@@ -56,6 +56,9 @@ struct GsBatch {
         int nbt = 0;
         int fx_shift = 0;
         int mst = 0;
+        int kind = 2;                    // 2: pair kernel, 3: triplet kernel
+        int nh = 0;                      // triplet histogram entries per region
+        int kmax = 0;                    // triplet neighbour capacity per apex
         DevBuf<unsigned char> blob;      // regions, bins and stencil rows read by the captured kernels
     };
     std::vector<Plan> plans;
@@ -99,5 +102,11 @@ std::vector<Row> full_stencil(double h, double rmax, int m);
 double ctx_xmax(gs_ctx* c);
 long double fx_value(unsigned long long lo, unsigned long long hi, int shift);
 int default_cell_div(const gs_ctx* c, double rmax, int mode);
+// 3PCF helpers shared by the single and batched paths (pcf34.cpp)
+std::vector<uint16_t> table3(int nb);
+std::vector<uint8_t> normal_table(int nm);
+struct TripArgs;
+int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted);
+void combine3(const long double* S, int nq, long double Wd, long double Wr, bool with_n3, double* n2, double* n3);
 
 }  // namespace gs

@@ -84,7 +84,10 @@ struct TripArgs {
     int32_t reps;
     int32_t kmax;
     int32_t lds_hist;           // histogram in LDS (else global atomics)
-    int32_t pad;
+    // batched regions (nreg > 0): each apex uses its region's grid (regions[r].lo/dim/cell_base,
+    // g.h shared) and adds to its region's histogram, out_cnt/out_w + r * [ncfg][nm][4]
+    int32_t nreg;
+    const RegionDesc* regions;
 };
 size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);

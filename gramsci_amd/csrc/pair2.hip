@@ -52,6 +52,18 @@ struct Stats {
     unsigned long long cand, rechk, over;
 };
 
+// Rotation of a register across the 64 lanes of a wavefront by one lane (DPP
+// wave_ror:1); every lane must be active.
+__device__ __forceinline__ int rot_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x134, 0xf, 0xf, false); }
+__device__ __forceinline__ uint32_t rot_lane(uint32_t v) { return (uint32_t)rot_lane((int)v); }
+__device__ __forceinline__ float rot_lane(float v) { return __int_as_float(rot_lane(__float_as_int(v))); }
+__device__ __forceinline__ double rot_lane(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = rot_lane((int)(b & 0xffffffffll)), hi = rot_lane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+
 // One pair's histogram update.  base = class offset + b*nm (flat bin index
 // without the mu bin); counts live at ((base + m) << rsh) + rep.
 template <bool RSD, bool W>
@@ -145,11 +157,9 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool 
     size_t b = 0;
     b += ((nbt * reps * 4 * copies + 15) / 16) * 16;          // counts
     if (weighted) b += nbt * 16 * copies;                     // split fixed-point sums
-    b += ((size_t)(kMaxBins + 1) * 8);                        // ok_lo/ok_hi as float2
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
     const int waves = kWG / kWave;
-    if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // x, y, z, w, flags, pad
-    else b += (size_t)waves * kWave * (16 + 8 + 4 + 4);       // float4, w, flags, pad
+    if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // LDS tile: x, y, z, w, flags, pad
     return b;
 }
 
@@ -179,28 +189,24 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     // the histogram this wave adds to: shared by the work-group, or wave-private in batched mode
     uint32_t* hc = batched ? hc0 + (size_t)wv * nbt : hc0;
     unsigned long long* hw = batched ? hw0 + (size_t)wv * 2 * nbt : hw0;
-    float2* okb = (float2*)p;
-    p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
-    // wave-private tiles
-    constexpr int kTileBytes = FP64 ? kWave * 40 : kWave * 32;
-    unsigned char* tb = p + (size_t)wv * kTileBytes;
-    double* tx = (double*)tb;            // FP64: x[64], y[64], z[64], w[64], flags[64]
+    // FP64: wave-private LDS tiles x[64], y[64], z[64], w[64], flags[64]
+    unsigned char* tb = p + (size_t)wv * kWave * 40;
+    double* tx = (double*)tb;
     double* ty = tx + kWave;
     double* tz = ty + kWave;
-    float4* tl = (float4*)tb;            // FP32: float4[64], w[64], flags[64]
-    double* tw = FP64 ? (tz + kWave) : (double*)(tl + kWave);
+    double* tw = tz + kWave;
     uint32_t* tf = (uint32_t*)(tw + kWave);
 
     for (int k = tid; k < nbt * reps * copies; k += kWG) hc0[k] = 0;
     if (W)
         for (int k = tid; k < 2 * nbt * copies; k += kWG) hw0[k] = 0;
-    for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
     for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
     __syncthreads();
 
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi, ua = B.ua, ub = B.ub;
+    const float ud0 = B.ud0, ud1 = B.ud1, ud2 = B.ud2, ud3 = B.ud3, nbf = (float)nb;
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const double ms = B.mu_scale;
     const int nbm = nb * nm;
@@ -282,84 +288,106 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
             const float zi_r = Li.z - (float)rw.dz * h32;
             for (int jt = js; jt < je; jt += kWave) {
                 const int nt = min(kWave, je - jt);
-                wave_sync();
-                if (lane < nt) {
-                    const int j = jt + lane;
-                    if constexpr (FP64) {
-                        const double4 Pj = a.pw[j];
-                        tx[lane] = Pj.x; ty[lane] = Pj.y; tz[lane] = Pj.z;
-                        if (W) tw[lane] = ldexp(Pj.w, shift);
-                    } else {
-                        tl[lane] = a.pl[j];
-                        if (W) tw[lane] = ldexp(a.pw[j].w, shift);
-                    }
-                    tf[lane] = a.flags[j];
-                }
-                wave_sync();
+                const int jmin = max(self ? i + 1 - jt : 0, 0);   // self row: only j > i
+                if (vi) st.cand += (unsigned long long)(nt - min(jmin, nt));
                 bound += (unsigned long long)nt * 2ull * kWave;
-                // rotated schedule: at step k lane L takes tile entry (k + L) mod nt, so the
-                // lanes of a wave pair with different neighbours (spreading their histogram
-                // updates over many bins) while reading consecutive, conflict-free LDS words
-                const int kend = vi ? nt : 0;
-                const int lm = lane % nt;
-                const int jmin = self ? i + 1 - jt : 0;   // self row: only j > i
-                for (int k = 0; k < kend; ++k) {
-                    int kk = k + lm;
-                    kk = kk >= nt ? kk - nt : kk;
-                    if (kk < jmin) continue;
-                    ++st.cand;
-                    int b, m = 0, mr = 0;
-                    const uint32_t fj = tf[kk];
-                    if constexpr (FP64) {
-                        const double jx = tx[kk], jy = ty[kk], jz = tz[kk];
-                        const double s = exact_s(Pi.x, Pi.y, Pi.z, jx, jy, jz);
-                        if (!(s > t_lo && s < t_hi)) continue;
-                        b = adjust_bin(s, bin_estimate((float)s, LOGB, ua, ub), t64, nb);
-                        if (b == nb) { st.over += (uint64_t)(ci + ((fj & 2) ? 0 : 1)); continue; }
-                        if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, jx, jy, jz, ms, nm, m, mr);
-                    } else {
-                        const float4 tj = tl[kk];
-                        const float dx = fmaf(tj.w - Li.w, h32, tj.x - Li.x);
-                        const float dy = tj.y - yi_r, dz = tj.z - zi_r;
-                        const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        if (!(s > rej_lo && s < rej_hi)) continue;
-                        b = (int)bin_estimate(s, LOGB, ua, ub);
-                        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
-                        const float2 ok = okb[b];
-                        bool safe = (s >= ok.x) && (s < ok.y);
-                        if constexpr (RSD) {
-                            // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
-                            const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
-                            const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
-                            const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
-                            const float rm = __builtin_amdgcn_rsqf(M2), rsq = __builtin_amdgcn_rsqf(s);
-                            const float t = fmaf(D * rm * rsq, ms32, ms32);     // (theta + 1) * mu_scale
-                            const float dt = ms32 * fmaf(mu_ew, rsq, fmaf(mu_em, rm, mu_c));
-                            // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
-                            // nearest integer is no edge or is farther than the error bound dt < 1/2
-                            const float rt = rintf(t);
-                            const bool edge = (rt >= 1.0f) && (rt <= (float)(nm - 1));
-                            safe = safe && (dt < 0.5f) && (!edge || fabsf(t - rt) >= dt);
-                            const int mi = (int)floorf(t);
-                            m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
-                            mr = nm - 1 - m;
-                        }
-                        if (!safe) {
-                            ++st.rechk;
-                            const double4 Pj = a.pw[jt + kk];
-                            Pi = a.pw[i];
-                            const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
-                            if (!(se > t_lo && se < t_hi)) continue;
-                            b = exact_bin(se, t64, nb);
-                            if (b == nb) { st.over += (uint64_t)(ci + ((fj & 2) ? 0 : 1)); continue; }
-                            if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
-                        }
-                    }
+                // histogram update of one kept pair with neighbour flags fj, bin b, mu bins m / mr
+                auto finish = [&](uint32_t fj, double wj, int b, int m, int mr) {
                     const int cj = (fj & 2) ? 0 : 1;
                     const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
                     double x = 0.0;
-                    if constexpr (W) x = wi * tw[kk];
+                    if constexpr (W) x = wi * wj;
                     accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt, a.out_w);
+                };
+                if constexpr (FP64) {
+                    wave_sync();
+                    if (lane < nt) {
+                        const double4 Pj = a.pw[jt + lane];
+                        tx[lane] = Pj.x; ty[lane] = Pj.y; tz[lane] = Pj.z;
+                        if (W) tw[lane] = ldexp(Pj.w, shift);
+                        tf[lane] = a.flags[jt + lane];
+                    }
+                    wave_sync();
+                    // rotated schedule: at step k lane L takes tile entry (k + L) mod nt, so the
+                    // lanes of a wave pair with different neighbours (spreading their histogram
+                    // updates over many bins) while reading consecutive, conflict-free LDS words
+                    const int kend = vi ? nt : 0;
+                    const int lm = lane % nt;
+                    for (int k = 0; k < kend; ++k) {
+                        int kk = k + lm;
+                        kk = kk >= nt ? kk - nt : kk;
+                        if (kk < jmin) continue;
+                        const uint32_t fj = tf[kk];
+                        const double jx = tx[kk], jy = ty[kk], jz = tz[kk];
+                        const double s = exact_s(Pi.x, Pi.y, Pi.z, jx, jy, jz);
+                        if (!(s > t_lo && s < t_hi)) continue;
+                        const int b = adjust_bin(s, bin_estimate((float)s, LOGB, ua, ub), t64, nb);
+                        if (b == nb) { st.over += (uint64_t)(ci + ((fj & 2) ? 0 : 1)); continue; }
+                        int m = 0, mr = 0;
+                        if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, jx, jy, jz, ms, nm, m, mr);
+                        finish(fj, W ? tw[kk] : 0.0, b, m, mr);
+                    }
+                } else {
+                    // Register tile: lane L loads entry L; the entries then rotate through the
+                    // wave one lane per step (DPP), so after 64 steps every lane has met every
+                    // entry.  Consecutive lanes meet different neighbours at each step, which
+                    // spreads their histogram updates over many bins, and no LDS is read.
+                    const bool has = lane < nt;
+                    const int jl = jt + (has ? lane : 0);
+                    float4 e = a.pl[jl];
+                    double ewj = 0.0;
+                    if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
+                    uint32_t ef = a.flags[jl];
+                    int eidx = has ? lane : -1;
+                    for (int k = 0; k < kWave; ++k) {
+                        if (vi && eidx >= jmin) do {
+                            const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
+                            const float dy = e.y - yi_r, dz = e.z - zi_r;
+                            const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                            if (!(s > rej_lo && s < rej_hi)) break;
+                            const float r = __builtin_amdgcn_rsqf(s);
+                            const float u = LOGB ? fmaf(__builtin_amdgcn_logf(s), ua, ub) : fmaf(s * r, ua, ub);
+                            const float du = fmaf(fmaf(ud2, r, ud1), r, fmaf(ud3, s * r, ud0));
+                            // the radial bin is certain when u is farther than du from every integer
+                            bool safe = fabsf(u - rintf(u)) >= du;
+                            if (safe && !(u >= 0.0f && u < nbf)) break;    // certainly outside (rmin, rmax)
+                            int b = (int)u;
+                            int m = 0, mr = 0;
+                            if constexpr (RSD) {
+                                // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
+                                const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
+                                const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
+                                const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
+                                const float rm = __builtin_amdgcn_rsqf(M2);
+                                const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
+                                const float dt = ms32 * fmaf(mu_ew, r, fmaf(mu_em, rm, mu_c));
+                                // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
+                                // nearest integer is no edge or is farther than the error bound dt < 1/2
+                                const float rt = rintf(t);
+                                const bool edge = (rt >= 1.0f) && (rt <= (float)(nm - 1));
+                                safe = safe && (dt < 0.5f) && (!edge || fabsf(t - rt) >= dt);
+                                const int mi = (int)floorf(t);
+                                m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
+                                mr = nm - 1 - m;
+                            }
+                            if (!safe) {
+                                ++st.rechk;
+                                const double4 Pj = a.pw[jt + eidx];
+                                Pi = a.pw[i];
+                                const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
+                                if (!(se > t_lo && se < t_hi)) break;
+                                b = exact_bin(se, t64, nb);
+                                if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); break; }
+                                if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                            }
+                            finish(ef, ewj, b, m, mr);
+                        } while (false);
+                        // rotate the register tile by one lane (all lanes active here)
+                        e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
+                        if constexpr (W) ewj = rot_lane(ewj);
+                        ef = rot_lane(ef);
+                        eidx = rot_lane(eidx);
+                    }
                 }
                 // keep every u32 LDS counter (and the split sums) below 2^32: each wave drains
                 // after 2^29 of its own increments, so four waves stay below 2^31 in total

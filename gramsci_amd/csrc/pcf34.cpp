@@ -27,7 +27,7 @@ void extra_release(GsExtra& x)
 
 // bintable for the 3PCF (gramsci.F90:1459-1474), 0-based configuration of a
 // sorted triple in the i <= j <= k enumeration, for every permutation.
-static std::vector<uint16_t> table3(int nb)
+std::vector<uint16_t> table3(int nb)
 {
     std::vector<uint16_t> t((size_t)nb * nb * nb, 0);
     int cfg = 0;
@@ -84,7 +84,7 @@ namespace gs {
 // assignment), the rest of the expression is single precision left to right,
 // and floor()+1 is stored in an int8 before the clamp.  A negative radicand
 // (NaN) ends in bin 1 whatever integer its floor converts to.
-static std::vector<uint8_t> normal_table(int nm)
+std::vector<uint8_t> normal_table(int nm)
 {
     std::vector<uint8_t> t((size_t)nm * nm, 0);
     const double mu_scale = (double)(0.5f * (float)nm);
@@ -105,6 +105,52 @@ static std::vector<uint8_t> normal_table(int nm)
             t[(size_t)(a - 1) * nm + (b - 1)] = (uint8_t)(v - 1);
         }
     return t;
+}
+
+// LDS plan of the triplet kernel for a.ncfg/a.nm: histogram replicas and
+// neighbour capacity within 64 KiB; a histogram too large for LDS goes to
+// global atomics instead.
+int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted)
+{
+    const size_t nh = (size_t)a.ncfg * a.nm * 4;
+    const size_t budget = 64 * 1024;
+    auto lds = [&]() { return triplet3_lds_bytes(a, nb, fp64, weighted); };
+    a.lds_hist = 1;
+    a.reps = 1;
+    while (a.reps < 32 && nh * a.reps * 2 * 4 <= 16 * 1024) a.reps *= 2;
+    a.kmax = 4096;
+    while (a.kmax > 512 && lds() > budget) a.kmax -= 256;
+    while (a.reps > 1 && lds() > budget) a.reps /= 2;
+    if (lds() > budget) {
+        a.lds_hist = 0;
+        a.reps = 1;
+        a.kmax = 4096;
+        while (a.kmax > 256 && lds() > budget) a.kmax -= 256;
+    }
+    if (lds() > budget) return fail(GS_ELIMIT, "3PCF tables do not fit in LDS");
+    return GS_OK;
+}
+
+// N2/N3 of the 3PCF from the class sums S[q][4] (q = configuration x mu bin):
+// N2 = sum over classes of (-1)^r S_r / (Wd^(3-r) Wr^r) (gramsci.F90:501, :507, :431, :437);
+// N3 = S_3 / Wr^3 (:498, :505, :434) unless with_n3 is false (the equilateral RSD
+// branch accumulates no N3, :429-431)
+void combine3(const long double* S, int nq, long double Wd, long double Wr, bool with_n3, double* n2, double* n3)
+{
+    for (int q = 0; q < nq; ++q) {
+        long double v2 = 0, v3 = 0;
+        for (int r = 0; r <= 3; ++r) {
+            const long double sv = S[q * 4 + r];
+            if (sv == 0) continue;
+            long double den = 1;
+            for (int t = 0; t < 3 - r; ++t) den *= Wd;
+            for (int t = 0; t < r; ++t) den *= Wr;
+            v2 += ((r & 1) ? -sv : sv) / den;
+        }
+        if (with_n3 && Wr > 0) v3 = S[q * 4 + 3] / (Wr * Wr * Wr);
+        if (n2) n2[q] = (double)v2;
+        if (n3) n3[q] = (double)v3;
+    }
 }
 
 // Shared body of gs_count_3pcf (all configurations) and gs_count_3pcf_equi.
@@ -139,23 +185,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     a.ncfg = ncfg;
     a.nm = nm;
     a.equi = equi ? 1 : 0;
-    // LDS plan: histogram replicas and neighbour capacity within 64 KiB; a
-    // histogram too large for LDS goes to global atomics instead
-    const size_t budget = 64 * 1024;
-    auto lds = [&]() { return triplet3_lds_bytes(a, nb, fp64, weighted); };
-    a.lds_hist = 1;
-    a.reps = 1;
-    while (a.reps < 32 && (size_t)nh * a.reps * 2 * 4 <= 16 * 1024) a.reps *= 2;
-    a.kmax = 4096;
-    while (a.kmax > 512 && lds() > budget) a.kmax -= 256;
-    while (a.reps > 1 && lds() > budget) a.reps /= 2;
-    if (lds() > budget) {
-        a.lds_hist = 0;
-        a.reps = 1;
-        a.kmax = 4096;
-        while (a.kmax > 256 && lds() > budget) a.kmax -= 256;
-    }
-    if (lds() > budget) return fail(GS_ELIMIT, "3PCF tables do not fit in LDS");
+    if ((rc = plan_triplet_lds(a, nb, fp64, weighted))) return rc;
 
     GsExtra& X = *v.extra;
     const size_t tab_off = ((sizeof(BinDesc) + 15) / 16) * 16;
@@ -216,24 +246,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
         S[k] = weighted ? fx_value(h_out[nh + 2 * k], h_out[nh + 2 * k + 1], B.fx_shift3) : (long double)h_out[k];
         if (wsum) wsum[k] = (double)S[k];
     }
-    // N2 = sum over classes of (-1)^r S_r / (Wd^(3-r) Wr^r) (gramsci.F90:501, :507, :431, :437);
-    // N3 = S_3 / Wr^3 (:498, :505, :434) -- the equilateral RSD branch has no N3 (:429-431)
-    const long double Wd = v.wsum_d, Wr = v.wsum_r;
-    const bool with_n3 = !(equi && nm > 1);
-    for (int q = 0; q < ncfg * nm; ++q) {
-        long double v2 = 0, v3 = 0;
-        for (int r = 0; r <= 3; ++r) {
-            const long double s = S[q * 4 + r];
-            if (s == 0) continue;
-            long double den = 1;
-            for (int t = 0; t < 3 - r; ++t) den *= Wd;
-            for (int t = 0; t < r; ++t) den *= Wr;
-            v2 += ((r & 1) ? -s : s) / den;
-        }
-        if (with_n3 && Wr > 0) v3 = S[q * 4 + 3] / (Wr * Wr * Wr);
-        if (n2) n2[q] = (double)v2;
-        if (n3) n3[q] = (double)v3;
-    }
+    combine3(S.data(), ncfg * nm, v.wsum_d, v.wsum_r, !(equi && nm > 1), n2, n3);
     fill_stats(st, v.ev, h_out.data() + 3 * nh, mst);
     return GS_OK;
 }

@@ -47,7 +47,7 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
     b += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;             // config table (u16)
     if (a.nm > 1) b += ((size_t)a.nm * a.nm + 15) / 16 * 16;    // find_normal table (u8)
     const size_t rec = fp64 ? 32 : 16;                          // coords + packed bin/class/mu
-    b += kmax * (rec + 4 + (weighted ? 8 : 0) + (a.nm > 1 ? 4 : 0));   // + sorted index, weight, orig
+    b += kmax * (rec + 4 + (weighted ? 8 : 0) + ((a.nm > 1 || weighted) ? 4 : 0));   // + sorted index, weight, orig
     return b;
 }
 
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     p += (size_t)kmax * 4;
     double* wl = (double*)p;             // neighbour weights * 2^shift3
     if (W) p += (size_t)kmax * 8;
-    int32_t* ol = (int32_t*)p;           // RSD: original index of each neighbour
+    int32_t* ol = (int32_t*)p;           // RSD or weighted: original index of each neighbour
 
     __shared__ int s_item, s_k, s_over;
 
@@ -114,8 +114,29 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
     const int shift = B.fx_shift3;
-    const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
     const double inv_h = 1.0 / a.g.h;
+    const bool batched = a.nreg > 0;
+    int cur_r = batched ? -1 : 0;   // region whose histogram the LDS holds (work-group uniform)
+
+    // adds the LDS histogram to region r's output and clears it (all threads of the work-group)
+    auto drain = [&](int r) {
+        unsigned long long* oc = a.out_cnt + (size_t)r * nh;
+        unsigned long long* ow = a.out_w + (size_t)r * 2 * nh;
+        for (int k = tid; k < nh; k += kWG) {
+            unsigned long long sum = 0;
+            for (int q = 0; q < reps; ++q) { sum += hc[k * reps + q]; hc[k * reps + q] = 0; }
+            if (sum) atomicAdd(&oc[k], sum);
+            if (W) {
+                const unsigned long long lo = hw[2 * k], hi = hw[2 * k + 1];
+                hw[2 * k] = 0; hw[2 * k + 1] = 0;
+                if (lo | hi) {
+                    const unsigned long long L = (hi << 32) + lo;
+                    const unsigned long long H = (hi >> 32) + (L < lo ? 1ull : 0ull);
+                    glb_add128(&ow[2 * k], L, H);
+                }
+            }
+        }
+    };
 
     unsigned long long units = 0, binned = 0, rechk = 0, over = 0, cand = 0;
     unsigned long long bound = 0;   // u32 increments since the last drain (uniform)
@@ -137,10 +158,32 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             __syncthreads();
             continue;
         }
+        // the apex's grid: the whole catalogue's, or its region's in a batch
+        int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2], cb = 0;
+        double lx = a.g.lo[0], ly = a.g.lo[1], lz = a.g.lo[2];
+        if (batched) {
+            int lo = 0, hi = a.nreg - 1;       // last region with pt_base <= apex
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (a.regions[mid].pt_base <= apex) lo = mid; else hi = mid - 1;
+            }
+            const RegionDesc& rd = a.regions[lo];
+            nx = rd.dim[0]; ny = rd.dim[1]; nz = rd.dim[2];
+            cb = rd.cell_base;
+            lx = rd.lo[0]; ly = rd.lo[1]; lz = rd.lo[2];
+            if (lo != cur_r) {
+                if (lh && cur_r >= 0) {
+                    drain(cur_r);
+                    bound = 0;
+                    __syncthreads();
+                }
+                cur_r = lo;
+            }
+        }
         const double4 Pa = a.pw[apex];
-        const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_h), 0), nx - 1);
-        const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), ny - 1);
-        const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nz - 1);
+        const int cx = min(max((int)floor((Pa.x - lx) * inv_h), 0), nx - 1);
+        const int cy = min(max((int)floor((Pa.y - ly) * inv_h), 0), ny - 1);
+        const int cz = min(max((int)floor((Pa.z - lz) * inv_h), 0), nz - 1);
 
         // ---- 1. gather the kept neighbours with their exact bin
         for (int r = 0; r < a.nrows; ++r) {
@@ -149,7 +192,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             if (by < 0 || by >= ny || bz < 0 || bz >= nz) continue;
             const int bx0 = max(cx + rw.x0, 0), bx1 = min(cx + rw.x1, nx - 1);
             if (bx0 > bx1) continue;
-            const int rb = (bz * ny + by) * nx;
+            const int rb = cb + (bz * ny + by) * nx;
             const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
             for (int j = js + tid; j < je; j += kWG) {
                 ++cand;
@@ -170,8 +213,8 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     int m, mr;
                     exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);   // apex -> j
                     pk |= (uint32_t)m << 9;
-                    ol[pos] = a.orig[j];
                 }
+                if (nm > 1 || W) ol[pos] = a.orig[j];
                 if constexpr (FP64) {
                     rec64[pos] = make_double4(Pj.x, Pj.y, Pj.z, (double)pk);
                 } else {
@@ -240,6 +283,13 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 }
             }
             if (equi && b3 != (int)(pu & 255u)) b3 = -1;
+            // the triple product in the reference's order, (w_i w_k1) w_k2 with k1 the neighbour of
+            // smaller original index (gramsci.F90:482, :501), so that sums do not depend on the
+            // order in which neighbours were gathered
+            auto triple = [&](int x, int y) {
+                const bool xf = ol[x] < ol[y];
+                return (wa * ldexp(wl[xf ? x : y], -shift)) * wl[xf ? y : x];
+            };
             if (b3 >= 0) {
                 const int cfg = equi ? b3 : (int)tab[((pu & 255u) * nb + (pv & 255u)) * nb + b3];
                 int mu = 0;
@@ -254,16 +304,17 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     atomicAdd(&hc[hk * reps + rep], 1u);
                     if constexpr (W) {
                         uint32_t qlo, qhi;
-                        quantise((wa * ldexp(wl[u], -shift)) * wl[v], qlo, qhi);
+                        quantise(triple(u, v), qlo, qhi);
                         atomicAdd(&hw[2 * hk], (unsigned long long)qlo);
                         atomicAdd(&hw[2 * hk + 1], (unsigned long long)qhi);
                     }
                 } else {
-                    atomicAdd(&a.out_cnt[hk], 1ull);
+                    const size_t go = (size_t)cur_r * nh + hk;
+                    atomicAdd(&a.out_cnt[go], 1ull);
                     if constexpr (W) {
                         uint32_t qlo, qhi;
-                        quantise((wa * ldexp(wl[u], -shift)) * wl[v], qlo, qhi);
-                        glb_add128(&a.out_w[2 * hk], ((unsigned long long)qhi << 32) + qlo, 0ull);
+                        quantise(triple(u, v), qlo, qhi);
+                        glb_add128(&a.out_w[2 * go], ((unsigned long long)qhi << 32) + qlo, 0ull);
                     }
                 }
             }
@@ -273,38 +324,13 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         bound += (unsigned long long)P;
         __syncthreads();
         if (lh && bound > (1ull << 31)) {
-            for (int k = tid; k < nh; k += kWG) {
-                unsigned long long s = 0;
-                for (int r = 0; r < reps; ++r) { s += hc[k * reps + r]; hc[k * reps + r] = 0; }
-                if (s) atomicAdd(&a.out_cnt[k], s);
-                if (W) {
-                    const unsigned long long lo = hw[2 * k], hi = hw[2 * k + 1];
-                    hw[2 * k] = 0; hw[2 * k + 1] = 0;
-                    if (lo | hi) {
-                        const unsigned long long L = (hi << 32) + lo;
-                        const unsigned long long H = (hi >> 32) + (L < lo ? 1ull : 0ull);
-                        glb_add128(&a.out_w[2 * k], L, H);
-                    }
-                }
-            }
+            drain(cur_r);
             bound = 0;
             __syncthreads();
         }
     }
     __syncthreads();
-    for (int k = tid; lh && k < nh; k += kWG) {
-        unsigned long long s = 0;
-        for (int r = 0; r < reps; ++r) s += hc[k * reps + r];
-        if (s) atomicAdd(&a.out_cnt[k], s);
-        if (W) {
-            const unsigned long long lo = hw[2 * k], hi = hw[2 * k + 1];
-            if (lo | hi) {
-                const unsigned long long L = (hi << 32) + lo;
-                const unsigned long long H = (hi >> 32) + (L < lo ? 1ull : 0ull);
-                glb_add128(&a.out_w[2 * k], L, H);
-            }
-        }
-    }
+    if (lh && cur_r >= 0) drain(cur_r);
     for (int off = 32; off > 0; off >>= 1) {
         units += __shfl_down(units, off, 64);
         binned += __shfl_down(binned, off, 64);

@@ -155,6 +155,19 @@ class Engine:
                                                 L.ptr(wsum), L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
         return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, xi=_xi(n2, n3), stats=st.as_dict())
 
+    def pcf3_batched(self, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):
+        """Per-region 3PCF of the batched catalogue: cnt[R, ncfg, nmu, 4], n2/n3[R, ncfg, nmu]."""
+        p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
+        R, nc = self.nreg, n3cfg(nbins)
+        cnt = np.zeros((R, nc, nmu, 4), np.uint64)
+        wsum = np.zeros((R, nc, nmu, 4))
+        n2 = np.zeros((R, nc, nmu))
+        n3 = np.zeros((R, nc, nmu))
+        st = L.gs_stats()
+        L.check(self._lib.gs_count_3pcf_batched(self._ctx, ctypes.byref(p), L.ptr(cnt, ctypes.c_uint64),
+                                                L.ptr(wsum), L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
+        return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, zeta=_xi(n2, n3), stats=st.as_dict())
+
     # ------------------------------------------------------------ counts
     @staticmethod
     def params(rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0):

@@ -155,6 +155,12 @@ int gs_set_catalogue_batched(gs_ctx* ctx, const double* xyz, const double* w, co
  * the captured pipeline for 2..6 cells per rmax (cached per parameter set). */
 int gs_count_2pcf_batched(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
                           double* n2, double* n3, gs_stats* st);
+/* Per-region 3PCF of the batched catalogue (query_graph_3pcf_all, gramsci.F90:463-545,
+ * on every region, in one captured launch of the triplet kernel):
+ * cnt/wsum [nreg][ncfg][nmu][4], n2/n3 [nreg][ncfg][nmu], as gs_count_3pcf per region.
+ * p->cell_div = 0 times 1..3 cells per rmax on the batch (cached per parameter set). */
+int gs_count_3pcf_batched(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
+                          double* n2, double* n3, gs_stats* st);
 
 /* Domain decomposition on the GPU, replacing the reference's offline tool
  * (domain_decomposition.f90:35-74).  region[i] in 1..nreg labels point i (other

@@ -69,3 +69,38 @@ def test_batched_equals_separate_launches_and_replays(eng, batch):
     c = eng.pcf2_batched(*args[:-1], args[-1], cell_div=5)
     assert np.array_equal(a["cnt"], c["cnt"]) and np.array_equal(a["wsum"], c["wsum"])
     del junk
+
+
+@pytest.mark.parametrize("prec,nmu,weighted", [("fp32", 1, False), ("fp64", 1, True), ("fp32", 3, True)])
+def test_batched_3pcf_matches_oracle_per_region(eng, batch, prec, nmu, weighted):
+    xyz, buf, off, ndata = batch
+    w = C.lognormal_weights(len(xyz), 0.2, 19) if weighted else None
+    p = O.Params(2.0, 8.0, 4, nmu, False)
+    eng.set_catalogue_batched(xyz, w, buf, off, ndata)
+    out = eng.pcf3_batched(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, prec, cell_div=2)
+    print("batched 3pcf", out["stats"], flush=True)
+    for r in range(len(ndata)):
+        sl = slice(off[r], off[r + 1])
+        ref = O.pcf3(xyz[sl], None if w is None else w[sl], int(ndata[r]), p, buf=buf[sl])
+        assert np.array_equal(out["cnt"][r].reshape(ref["cnt"].shape), ref["cnt"]), r
+        assert _close(out["n2"][r].reshape(-1), ref["n2"].reshape(-1))
+        assert _close(out["n3"][r].reshape(-1), ref["n3"].reshape(-1))
+
+
+def test_batched_3pcf_equals_separate_launches_and_replays(eng, batch):
+    xyz, buf, off, ndata = batch
+    w = C.lognormal_weights(len(xyz), 0.2, 23)
+    args = (2.0, 8.0, 4, 1, False, "fp32")
+    sep = []
+    for r in range(len(ndata)):
+        sl = slice(off[r], off[r + 1])
+        eng.set_catalogue(xyz[sl], w[sl], int(ndata[r]), buf[sl])
+        sep.append(eng.pcf3(*args[:3], nmu=1, precision="fp32"))
+    eng.set_catalogue_batched(xyz, w, buf, off, ndata)
+    a = eng.pcf3_batched(*args)                     # autotuned cell size
+    for rep in range(3):
+        b = eng.pcf3_batched(*args)
+        assert np.array_equal(a["cnt"], b["cnt"]) and np.array_equal(a["wsum"], b["wsum"]), rep
+    for r in range(len(ndata)):
+        assert np.array_equal(a["cnt"][r][:, 0], sep[r]["cnt"]), r
+        assert np.array_equal(a["wsum"][r][:, 0], sep[r]["wsum"]), r

@@ -4,6 +4,9 @@
 #include "../../include/gramsci_hip.h"
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -225,37 +228,6 @@ int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc
         const double inv = 1.0 / ((p.rmax - p.rmin) / nb);
         B.ua = (float)inv;
         B.ub = (float)(-p.rmin * inv);
-    }
-    // radial guard of the FP32 pair path in bin units.  The kernel's bin
-    // coordinate is u = fl(fl(log2 s32)*ua + ub) (log bins) or
-    // u = fl(fl(s32*rsq(s32))*ua + ub) (linear bins), against U(s) of the exact
-    // s, whose integer crossings are the thresholds t_k (to ~1e-13).  With
-    // |s32 - s| <= g(s) (s_guard), the bound is |dU/ds| g(s) + the log2 /
-    // square-root error (taken as 2^-16 absolute and 2^-20 relative, far above
-    // the hardware's ~1 ulp) + the fma rounding + 1e-9, times 1.25:
-    //   log:    dU/ds = ua/(s ln2)
-    //   linear: dU/ds = ua/(2 sqrt(s))
-    {
-        const double r3 = 2.0 * std::sqrt(3.0);
-        const double fma_err = std::ldexp((double)nb + 2.0, -23) + 1e-9;
-        double d0, d1, d2, d3;
-        if (p.log_bins) {
-            const double k = 2.0 * (double)B.ua / std::log(2.0);
-            d0 = k * std::ldexp(1.0, -22) + (double)B.ua * std::ldexp(1.0, -16) + fma_err;
-            d1 = k * r3 * ew;
-            d2 = k * 3.0 * ew * ew;
-            d3 = 0.0;
-        } else {
-            const double ua = B.ua;
-            d0 = ua * r3 * ew + ua * std::ldexp(1.0, -16) + fma_err;
-            d1 = ua * 3.0 * ew * ew;
-            d2 = 0.0;
-            d3 = ua * (std::ldexp(1.0, -22) + std::ldexp(1.0, -20));
-        }
-        B.ud0 = f_up(1.25 * d0);
-        B.ud1 = f_up(1.25 * d1);
-        B.ud2 = f_up(1.25 * d2);
-        B.ud3 = f_up(1.25 * d3);
     }
     // mu guard: per-component errors of sep (ew) and of the absolute midpoint (em)
     const double em = std::ldexp(2.0 * xmax + 3.0 * rrel, -24) * 1.01;
@@ -610,7 +582,7 @@ int default_cell_div(const gs_ctx* c, double rmax, int mode)
     int best = 1;
     for (int m = 1; m <= 8; ++m) {
         const double h = rmax / m;
-        if (dens * h * h * h >= 160.0) best = m;
+        if (dens * h * h * h >= 48.0) best = m;
     }
     return best;
 }
@@ -644,20 +616,47 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     BinDesc B;
     if ((rc = fill_bins(*p, (mst + 1) * h, xmax, c->wmax, B))) return rc;
 
-    // LDS histogram replicas (spreading same-bin atomics): the most replicas, at
-    // most 32, that keep the largest number of resident work-groups per CU
-    int reps = 1, best_occ = 0;
-    for (int r = 1; r <= 32; r *= 2) {
-        if (pair2_lds_bytes(nb, rsd ? nm : 1, r, fp64, weighted) > 64 * 1024) break;
-        const int occ = pair2_occupancy(nb, nm, r, fp64, rsd, weighted, p->log_bins != 0);
-        if (occ >= best_occ) { best_occ = occ; reps = r; }
+    // launch shape: work-group size and LDS histogram replicas (spreading same-bin
+    // atomics).  256-thread work-groups (four waves per shared histogram) are kept
+    // when they reach 16 resident waves per CU; otherwise larger work-groups share
+    // one histogram among more waves to raise residency.  Among shapes with the
+    // same residency, more replicas win.
+    int reps = 1, wg = kWG, best_waves = 0;
+    {
+        const char* ev_reps = getenv("GS_PAIR_REPS");    // tuning overrides (unset or empty: automatic)
+        const char* ev_wg = getenv("GS_PAIR_WG");
+        if (ev_reps && !*ev_reps) ev_reps = nullptr;
+        if (ev_wg && !*ev_wg) ev_wg = nullptr;
+        char key[96];
+        snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%s/%s", nb, nm, (int)fp64, (int)rsd, (int)weighted,
+                 (int)(p->log_bins != 0), ev_reps ? ev_reps : "", ev_wg ? ev_wg : "");
+        static std::mutex mu;
+        static std::map<std::string, std::array<int, 3>> shapes;
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = shapes.find(key);
+        if (it != shapes.end()) {
+            reps = it->second[0]; wg = it->second[1]; best_waves = it->second[2];
+        } else {
+            for (int pass = 0; pass < 2 && (pass == 0 || best_waves < 16); ++pass) {
+                for (int r = 1; r <= 32; r *= 2) {
+                    for (int w = pass == 0 ? 256 : 384; w <= (pass == 0 ? 256 : 1024); w += 128) {
+                        if (ev_reps && atoi(ev_reps) != r) continue;
+                        if (ev_wg && atoi(ev_wg) != w) continue;
+                        if (pair2_lds_bytes(nb, rsd ? nm : 1, r, fp64, weighted, false, w) > 160 * 1024) continue;
+                        const int waves =
+                            pair2_occupancy(nb, nm, r, fp64, rsd, weighted, p->log_bins != 0, false, w) * (w / kWave);
+                        if (waves > best_waves || (waves == best_waves && r > reps && w == wg)) {
+                            best_waves = waves;
+                            reps = r;
+                            wg = w;
+                        }
+                    }
+                }
+            }
+            shapes[key] = {reps, wg, best_waves};
+        }
     }
-    if (const char* ev = getenv("GS_PAIR_REPS")) {   // tuning override (power of two)
-        int v = atoi(ev);
-        if (v >= 1 && v <= 32 && (v & (v - 1)) == 0) reps = v;
-    }
-    if (pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted) > 64 * 1024)
-        return fail(GS_ELIMIT, "histogram of %d bins does not fit in LDS", nbt);
+    if (best_waves == 0) return fail(GS_ELIMIT, "histogram of %d bins does not fit in LDS", nbt);
 
     const size_t nout = (size_t)nbt + 2 * (size_t)nbt + 4;
     if ((rc = c->params.reserve(sizeof(BinDesc) + rows.size() * sizeof(Row) + 64)) || (rc = c->out.reserve(nout)) ||
@@ -690,12 +689,12 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     a.n_chunks = c->n_chunks;
     a.reps = reps;
     a.mirror = (rsd && !c->has_buf) ? 1 : 0;
-    const int occ = pair2_occupancy(nb, nm, reps, fp64, rsd, weighted, p->log_bins != 0);
+    const int occ = best_waves / (wg / kWave);
     int nblocks = c->n_cu * occ;
     if (nblocks > c->n_chunks) nblocks = std::max(c->n_chunks, 1);
     (void)hipEventRecord(c->ev[2], c->stream);
     if (c->n_chunks > 0) {
-        e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p->log_bins != 0, nblocks, c->stream);
+        e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p->log_bins != 0, nblocks, wg, c->stream);
         if (e != hipSuccess) return hip_fail(e, "pair kernel launch");
     }
     (void)hipEventRecord(c->ev[3], c->stream);

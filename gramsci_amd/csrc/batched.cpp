@@ -216,7 +216,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         ta.regions = d_reg;
     }
     const int occ = kind == 3 ? triplet3_occupancy(ta, nb, fp64, weighted)
-                              : pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true);
+                              : std::max(1, pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true));
     const int nblocks = kind == 3 ? std::max(1, (int)std::min<int64_t>((int64_t)v.n_cu * occ, std::max<int64_t>(n, 1)))
                                   : v.n_cu * occ;
 
@@ -236,7 +236,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         if (e == hipSuccess)
             e = launch_row_chunks_b(B.cell_start.ptr, d_reg, a.row_base, nreg, (int)rows_tot, B.nch.ptr, v.stream);
         if (e == hipSuccess) e = exclusive_scan(B.nch.ptr, B.row_chunk.ptr, rows_tot + 1, B.scan_tmp.ptr, v.stream);
-        if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p.log_bins != 0, nblocks, v.stream);
+        if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p.log_bins != 0, nblocks, kWG, v.stream);
     } else if (e == hipSuccess && n > 0) {
         e = launch_triplet3(ta, nb, fp64, weighted, nblocks, v.stream);
     }

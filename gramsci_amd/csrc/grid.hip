@@ -136,7 +136,8 @@ int64_t scan_scratch_size(int64_t n)
 }
 
 // ---------------------------------------------------------------- scatter
-// Writes the points in cell order.  Order inside a cell follows the atomic
+// Writes the points in cell order, a cell's data points before its randoms
+// (consecutive neighbours then mostly share a class).  Order inside a cell otherwise follows the atomic
 // slot order; every count and fixed-point sum computed from the sorted arrays
 // is independent of it.
 __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz, const double* __restrict__ w,
@@ -144,10 +145,11 @@ __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz,
                                                  const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
                                                  int32_t* __restrict__ fill, double4* __restrict__ pw,
                                                  float4* __restrict__ pl, uint32_t* __restrict__ flags,
-                                                 int32_t* __restrict__ orig)
+                                                 int32_t* __restrict__ orig, int pass)
 {
     const int nx = g.dim[0], ny = g.dim[1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if ((i >= n_data) != (pass == 1)) continue;    // pass 0: data points, pass 1: randoms
         const int c = cell_of[i];
         const int pos = start[c] + atomicAdd(&fill[c], 1);
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
@@ -205,8 +207,10 @@ hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf
     int nblk = (int)((n + 255) / 256);
     if (nblk > 65535) nblk = 65535;
     if (nblk < 1) nblk = 1;
-    hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, st, xyz, w, buf, n, n_data, g, cell_of, start, fill, pw,
-                       pl, flags, orig);
+    // two passes on one stream: a cell's data points take its first slots, its randoms the rest
+    for (int pass = 0; pass < 2; ++pass)
+        hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, st, xyz, w, buf, n, n_data, g, cell_of, start, fill,
+                           pw, pl, flags, orig, pass);
     return hipGetLastError();
 }
 
@@ -245,11 +249,12 @@ __global__ __launch_bounds__(256) void k_scatter_b(const double* __restrict__ xy
                                                    const int32_t* __restrict__ cell_of,
                                                    const int32_t* __restrict__ start, int32_t* __restrict__ fill,
                                                    double4* __restrict__ pw, float4* __restrict__ pl,
-                                                   uint32_t* __restrict__ flags, int32_t* __restrict__ orig)
+                                                   uint32_t* __restrict__ flags, int32_t* __restrict__ orig, int pass)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int r = region_of(off, nreg, i);
         const RegionDesc R = reg[r];
+        if (((i - R.pt_base) >= R.n_data) != (pass == 1)) continue;   // pass 0: data points, pass 1: randoms
         const int c = cell_of[i];
         const int pos = start[c] + atomicAdd(&fill[c], 1);
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
@@ -314,8 +319,9 @@ hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* b
                             int nreg, const RegionDesc* reg, double h, const int32_t* cell_of, const int32_t* start,
                             int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_scatter_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, w, buf, n, off, nreg, reg, h, cell_of,
-                       start, fill, pw, pl, flags, orig);
+    for (int pass = 0; pass < 2; ++pass)
+        hipLaunchKernelGGL(k_scatter_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, w, buf, n, off, nreg, reg, h,
+                           cell_of, start, fill, pw, pl, flags, orig, pass);
     return hipGetLastError();
 }
 

@@ -37,7 +37,6 @@ struct BinDesc {
     float   ok_hi[kMaxBins + 1];
     float   rej_lo, rej_hi;        // s32 <= rej_lo or s32 >= rej_hi: certainly not kept
     float   ua, ub;                // approximate bin: u = f(s)*ua + ub, f = sqrt (linear) or log2 (log)
-    float   ud0, ud1, ud2, ud3;    // radial guard in bin units: |u - U(s)| <= ud0 + ud1*r + ud2*r^2 + ud3*s*r, r = 1/sqrt(s)
     float   ms32;                  // mu_scale in FP32
     float   mu_ew, mu_em, mu_c;    // mu guard: dt = ms*(mu_ew/|sep| + mu_em/|mid| + mu_c)
     // weighted fixed point: q = rint(w_i*w_j*... * 2^fx_shift)

@@ -147,11 +147,12 @@ hipError_t launch_adj_partials(const double* G, const double* w, int64_t n, int6
 hipError_t launch_adj_finish(const double* G, int64_t n, int64_t n_data, double s_d, double s_r, double inv_wd,
                              double inv_wr, double* out, hipStream_t st);
 
-// Dynamic LDS bytes of the pair kernel for these parameters.
-size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched = false);
+// Dynamic LDS bytes of the pair kernel for these parameters and work-group size wg (a multiple of 64, <= 1024).
+size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched = false, int wg = kWG);
 hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, bool logb,
-                        int nblocks, hipStream_t st);
-// Work-groups per CU the pair kernel can keep resident for this LDS size.
-int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool logb, bool batched = false);
+                        int nblocks, int wg, hipStream_t st);
+// Work-groups of wg threads per CU the pair kernel can keep resident (0 if it cannot run).
+int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool logb, bool batched = false,
+                    int wg = kWG);
 
 }  // namespace gs

@@ -65,31 +65,29 @@ __device__ __forceinline__ double rot_lane(double v)
 }
 
 // One pair's histogram update.  base = class offset + b*nm (flat bin index
-// without the mu bin); counts live at ((base + m) << rsh) + rep.
+// without the mu bin); counts live at ((base + m) << rsh) + rep.  Returns the
+// number of LDS bin increments made.
 template <bool RSD, bool W>
-__device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw, int rsh, int rep, int nm,
-                                           bool mirror, int base, int m, int mr, int ci, int cj, double x,
-                                           unsigned long long* out_cnt, unsigned long long* out_w)
+__device__ __forceinline__ int accumulate(uint32_t* hc, unsigned long long* hw, int rsh, int rep, int nm,
+                                          bool mirror, int base, int m, int mr, int ci, int cj, double x,
+                                          unsigned long long* out_cnt, unsigned long long* out_w)
 {
     uint32_t qlo = 0, qhi = 0;
     if constexpr (W) quantise(x, qlo, qhi);
+    auto add = [&](int k, uint32_t c) {
+        atomicAdd(&hc[(k << rsh) + rep], c);
+        if constexpr (W) {
+            atomicAdd(&hw[2 * k], (unsigned long long)qlo * c);
+            atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi * c);
+        }
+    };
     if constexpr (!RSD) {
         const int c = ci + cj;
-        if (c) {
-            atomicAdd(&hc[(base << rsh) + rep], (uint32_t)c);
-            if constexpr (W) {
-                atomicAdd(&hw[2 * base], (unsigned long long)qlo * c);
-                atomicAdd(&hw[2 * base + 1], (unsigned long long)qhi * c);
-            }
-        }
+        if (c) add(base, (uint32_t)c);
+        return c;
     } else if (mirror && mr == nm - 1 - m) {
-        // mirror-implied histogram: this update stands for i->j at m and j->i at nm-1-m
-        const int k = base + m;
-        atomicAdd(&hc[(k << rsh) + rep], 1u);
-        if constexpr (W) {
-            atomicAdd(&hw[2 * k], (unsigned long long)qlo);
-            atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi);
-        }
+        add(base + m, 1u);    // mirror-implied: stands for i->j at m and j->i at nm-1-m
+        return 1;
     } else if (mirror) {
         // a pair exactly on a mu edge: both ordered pairs explicitly, straight to global memory (rare)
         const unsigned long long L = ((unsigned long long)qhi << 32) + qlo;
@@ -99,29 +97,17 @@ __device__ __forceinline__ void accumulate(uint32_t* hc, unsigned long long* hw,
             glb_add128(&out_w[2 * (base + m)], L, 0ull);
             glb_add128(&out_w[2 * (base + mr)], L, 0ull);
         }
+        return 0;
     } else {
-        if (ci) {
-            const int k = base + m;
-            atomicAdd(&hc[(k << rsh) + rep], 1u);
-            if constexpr (W) {
-                atomicAdd(&hw[2 * k], (unsigned long long)qlo);
-                atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi);
-            }
-        }
-        if (cj) {
-            const int k = base + mr;
-            atomicAdd(&hc[(k << rsh) + rep], 1u);
-            if constexpr (W) {
-                atomicAdd(&hw[2 * k], (unsigned long long)qlo);
-                atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi);
-            }
-        }
+        if (ci) add(base + m, 1u);
+        if (cj) add(base + mr, 1u);
+        return ci + cj;
     }
 }
 
 // Drains the LDS histogram into the global accumulators with atomic
 // exchanges, so any wave may do it while the others keep adding.  Called by
-// all lanes of one wave (stride 64) or of the whole work-group (stride 256).
+// all lanes of one wave (stride 64) or of the whole work-group.
 // With `mirror`, entry (cls, b, m) also stands for (cls, b, nm-1-m).
 __device__ void drain(uint32_t* hc, unsigned long long* hw, int nbt, int nm, int reps, bool weighted, bool mirror,
                       int t0, int stride, unsigned long long* out_cnt, unsigned long long* out_w)
@@ -148,23 +134,30 @@ __device__ void drain(uint32_t* hc, unsigned long long* hw, int nbt, int nm, int
     }
 }
 
+__device__ __forceinline__ int wave_sum(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 }  // namespace
 
-size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched)
+size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool batched, int wg)
 {
     const size_t nbt = (size_t)3 * nb * nm;
-    const size_t copies = batched ? (size_t)(kWG / kWave) : 1;   // batched: one histogram per wave
+    const size_t copies = batched ? (size_t)(wg / kWave) : 1;   // batched: one histogram per wave
     size_t b = 0;
     b += ((nbt * reps * 4 * copies + 15) / 16) * 16;          // counts
     if (weighted) b += nbt * 16 * copies;                     // split fixed-point sums
+    b += ((size_t)(kMaxBins + 1) * 8);                        // ok_lo/ok_hi as float2
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
-    const int waves = kWG / kWave;
+    const int waves = wg / kWave;
     if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // LDS tile: x, y, z, w, flags, pad
     return b;
 }
 
 template <bool FP64, bool RSD, bool W, bool LOGB>
-__global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
+__global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const BinDesc& B = *a.bins;
@@ -179,7 +172,8 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     const int rep = lane & (reps - 1);
     const int rsh = 31 - __clz(reps);
     const bool mirror = RSD && a.mirror;
-    const int copies = batched ? kWG / kWave : 1;
+    const int nthr = blockDim.x;        // a multiple of 64, at most 1024
+    const int copies = batched ? nthr / kWave : 1;
 
     unsigned char* p = smem;
     uint32_t* hc0 = (uint32_t*)p;
@@ -189,6 +183,8 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     // the histogram this wave adds to: shared by the work-group, or wave-private in batched mode
     uint32_t* hc = batched ? hc0 + (size_t)wv * nbt : hc0;
     unsigned long long* hw = batched ? hw0 + (size_t)wv * 2 * nbt : hw0;
+    float2* okb = (float2*)p;
+    p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
     // FP64: wave-private LDS tiles x[64], y[64], z[64], w[64], flags[64]
@@ -199,14 +195,14 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     double* tw = tz + kWave;
     uint32_t* tf = (uint32_t*)(tw + kWave);
 
-    for (int k = tid; k < nbt * reps * copies; k += kWG) hc0[k] = 0;
+    for (int k = tid; k < nbt * reps * copies; k += nthr) hc0[k] = 0;
     if (W)
-        for (int k = tid; k < 2 * nbt * copies; k += kWG) hw0[k] = 0;
-    for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
+        for (int k = tid; k < 2 * nbt * copies; k += nthr) hw0[k] = 0;
+    for (int k = tid; k < nb; k += nthr) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
+    for (int k = tid; k < nb + 2; k += nthr) t64[k] = B.t64[k];
     __syncthreads();
 
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi, ua = B.ua, ub = B.ub;
-    const float ud0 = B.ud0, ud1 = B.ud1, ud2 = B.ud2, ud3 = B.ud3, nbf = (float)nb;
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const double ms = B.mu_scale;
     const int nbm = nb * nm;
@@ -218,7 +214,41 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
     int cur_r = -1;                 // region of this wave's private histogram (batched mode)
 
     Stats st{0, 0, 0};
+    // FP32 weighted path: per-lane run cache of the last histogram bin.  A lane's consecutive
+    // neighbours (adjacent points of one cell, data before randoms) often fall in the same
+    // bin; their updates are merged in registers and reach the LDS histogram with one set of
+    // atomics when the bin changes, which relieves same-address LDS atomic conflicts.
+    int ck = -1;
+    uint32_t cc = 0;
+    unsigned long long clo = 0, chi = 0;
+    auto flush = [&]() {
+        if (ck >= 0) {
+            atomicAdd(&hc[(ck << rsh) + rep], cc);
+            if constexpr (W) {
+                atomicAdd(&hw[2 * ck], clo);
+                atomicAdd(&hw[2 * ck + 1], chi);
+            }
+            ck = -1;
+        }
+    };
+    auto put = [&](int k, uint32_t c, uint32_t qlo, uint32_t qhi) {
+        if (k != ck) {
+            flush();
+            ck = k;
+            cc = 0;
+            clo = 0;
+            chi = 0;
+        }
+        cc += c;
+        if constexpr (W) {
+            clo += (unsigned long long)qlo * c;
+            chi += (unsigned long long)qhi * c;
+        }
+    };
     unsigned long long bound = 0;   // LDS increments issued by this wave since its last drain
+    // LDS bin increments between drains, over all waves sharing the histogram, stay below 2^31,
+    // which keeps every u32 count and every split sum from overflowing
+    const unsigned long long drain_at = (1ull << 31) / (unsigned long long)(batched ? 1 : nthr / kWave);
     const int xcd = blockIdx.x & 7;
     int tried = 0;
 
@@ -238,6 +268,7 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
             cb = rd.cell_base;
             row = grow - rd.row_base;
             if (R != cur_r) {
+                flush();
                 if (cur_r >= 0)
                     drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + (size_t)cur_r * nbt,
                           a.out_w + (size_t)cur_r * 2 * nbt);
@@ -290,14 +321,15 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                 const int nt = min(kWave, je - jt);
                 const int jmin = max(self ? i + 1 - jt : 0, 0);   // self row: only j > i
                 if (vi) st.cand += (unsigned long long)(nt - min(jmin, nt));
-                bound += (unsigned long long)nt * 2ull * kWave;
+                int nup = 0;    // LDS bin increments of this lane in this tile
                 // histogram update of one kept pair with neighbour flags fj, bin b, mu bins m / mr
                 auto finish = [&](uint32_t fj, double wj, int b, int m, int mr) {
                     const int cj = (fj & 2) ? 0 : 1;
                     const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
                     double x = 0.0;
                     if constexpr (W) x = wi * wj;
-                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt, a.out_w);
+                    nup += accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt,
+                                              a.out_w);
                 };
                 if constexpr (FP64) {
                     wave_sync();
@@ -331,7 +363,8 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                     // Register tile: lane L loads entry L; the entries then rotate through the
                     // wave one lane per step (DPP), so after 64 steps every lane has met every
                     // entry.  Consecutive lanes meet different neighbours at each step, which
-                    // spreads their histogram updates over many bins, and no LDS is read.
+                    // spreads their histogram updates over many bins, and no LDS is read for
+                    // the neighbours.
                     const bool has = lane < nt;
                     const int jl = jt + (has ? lane : 0);
                     float4 e = a.pl[jl];
@@ -345,15 +378,13 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                             const float dy = e.y - yi_r, dz = e.z - zi_r;
                             const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                             if (!(s > rej_lo && s < rej_hi)) break;
-                            const float r = __builtin_amdgcn_rsqf(s);
-                            const float u = LOGB ? fmaf(__builtin_amdgcn_logf(s), ua, ub) : fmaf(s * r, ua, ub);
-                            const float du = fmaf(fmaf(ud2, r, ud1), r, fmaf(ud3, s * r, ud0));
-                            // the radial bin is certain when u is farther than du from every integer
-                            bool safe = fabsf(u - rintf(u)) >= du;
-                            if (safe && !(u >= 0.0f && u < nbf)) break;    // certainly outside (rmin, rmax)
-                            int b = (int)u;
+                            int b = (int)bin_estimate(s, LOGB, ua, ub);
+                            b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                            const float2 ok = okb[b];
+                            bool safe = (s >= ok.x) && (s < ok.y);
                             int m = 0, mr = 0;
                             if constexpr (RSD) {
+                                const float r = __builtin_amdgcn_rsqf(s);
                                 // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
                                 const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
                                 const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
@@ -380,7 +411,22 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                                 if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); break; }
                                 if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
                             }
-                            finish(ef, ewj, b, m, mr);
+                            {
+                                const int cj = (ef & 2) ? 0 : 1;
+                                const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
+                                uint32_t qlo = 0, qhi = 0;
+                                if constexpr (W) quantise(wi * ewj, qlo, qhi);
+                                if constexpr (W && !RSD) {
+                                    if (ci + cj) put(base, (uint32_t)(ci + cj), qlo, qhi);
+                                    nup += ci + cj;
+                                } else if (W && mirror && mr == nm - 1 - m) {
+                                    put(base + m, 1u, qlo, qhi);   // stands for i->j at m and j->i at nm-1-m
+                                    ++nup;
+                                } else {
+                                    nup += accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj,
+                                                              wi * ewj, a.out_cnt, a.out_w);
+                                }
+                            }
                         } while (false);
                         // rotate the register tile by one lane (all lanes active here)
                         e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
@@ -389,9 +435,11 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
                         eidx = rot_lane(eidx);
                     }
                 }
-                // keep every u32 LDS counter (and the split sums) below 2^32: each wave drains
-                // after 2^29 of its own increments, so four waves stay below 2^31 in total
-                if (bound > (1ull << 29)) {
+                // each wave drains after drain_at of its own increments (the increments held in the
+                // lanes' run caches included), which bounds every LDS sum
+                bound += (unsigned long long)wave_sum(nup);
+                if (bound > drain_at) {
+                    flush();
                     const size_t ro = batched ? (size_t)cur_r : 0;
                     drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + ro * nbt,
                           a.out_w + ro * 2 * nbt);
@@ -400,13 +448,14 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
             }
         }
     }
+    flush();
     if (batched) {
         if (cur_r >= 0)
             drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + (size_t)cur_r * nbt,
                   a.out_w + (size_t)cur_r * 2 * nbt);
     } else {
         __syncthreads();
-        drain(hc, hw, nbt, nm, reps, W, mirror, tid, kWG, a.out_cnt, a.out_w);
+        drain(hc, hw, nbt, nm, reps, W, mirror, tid, nthr, a.out_cnt, a.out_w);
     }
 
     // wave-reduce the statistics
@@ -423,24 +472,28 @@ __global__ __launch_bounds__(kWG) void k_pair2(PairArgs a)
 }
 
 template <bool FP64, bool RSD, bool W, bool LOGB>
-static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, hipStream_t st)
+static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, int wg, hipStream_t st)
 {
     auto kern = k_pair2<FP64, RSD, W, LOGB>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kWG), lds, st, a);
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(wg), lds, st, a);
     return hipGetLastError();
 }
 
 template <bool FP64, bool RSD, bool W, bool LOGB>
-static int occ_t(size_t lds)
+static int occ_t(size_t lds, int wg)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pair2<FP64, RSD, W, LOGB>, kWG, lds) != hipSuccess)
+    auto kern = k_pair2<FP64, RSD, W, LOGB>;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, wg, lds) != hipSuccess)
         return 1;
-    return nb > 0 ? nb : 1;
+    return nb;
 }
 
 // Runs F<FP64, RSD, W, LOGB>(args...) for the runtime flags.
@@ -460,27 +513,28 @@ static auto dispatch(bool fp64, bool rsd, bool w, bool logb, Args... args)
 
 template <bool A, bool B, bool C, bool D>
 struct Launch {
-    static hipError_t run(const PairArgs& a, size_t lds, int nblocks, hipStream_t st)
+    static hipError_t run(const PairArgs& a, size_t lds, int nblocks, int wg, hipStream_t st)
     {
-        return launch_t<A, B, C, D>(a, lds, nblocks, st);
+        return launch_t<A, B, C, D>(a, lds, nblocks, wg, st);
     }
 };
 template <bool A, bool B, bool C, bool D>
 struct Occ {
-    static int run(size_t lds) { return occ_t<A, B, C, D>(lds); }
+    static int run(size_t lds, int wg) { return occ_t<A, B, C, D>(lds, wg); }
 };
 
 hipError_t launch_pair2(const PairArgs& a, int nb, int nm, bool fp64, bool rsd, bool weighted, bool logb,
-                        int nblocks, hipStream_t st)
+                        int nblocks, int wg, hipStream_t st)
 {
-    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, a.reps, fp64, weighted, a.nreg > 0);
-    return dispatch<Launch>(fp64, rsd, weighted, logb, a, lds, nblocks, st);
+    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, a.reps, fp64, weighted, a.nreg > 0, wg);
+    return dispatch<Launch>(fp64, rsd, weighted, logb, a, lds, nblocks, wg, st);
 }
 
-int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool logb, bool batched)
+int pair2_occupancy(int nb, int nm, int reps, bool fp64, bool rsd, bool weighted, bool logb, bool batched, int wg)
 {
-    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted, batched);
-    return dispatch<Occ>(fp64, rsd, weighted, logb, lds);
+    const size_t lds = pair2_lds_bytes(nb, rsd ? nm : 1, reps, fp64, weighted, batched, wg);
+    if (lds > 160 * 1024) return 0;
+    return dispatch<Occ>(fp64, rsd, weighted, logb, lds, wg);
 }
 
 }  // namespace gs

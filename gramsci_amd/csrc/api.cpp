@@ -582,7 +582,7 @@ int default_cell_div(const gs_ctx* c, double rmax, int mode)
     int best = 1;
     for (int m = 1; m <= 8; ++m) {
         const double h = rmax / m;
-        if (dens * h * h * h >= 48.0) best = m;
+        if (dens * h * h * h >= 32.0) best = m;
     }
     return best;
 }

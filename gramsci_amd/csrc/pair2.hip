@@ -134,11 +134,6 @@ __device__ void drain(uint32_t* hc, unsigned long long* hw, int nbt, int nm, int
     }
 }
 
-__device__ __forceinline__ int wave_sum(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 }  // namespace
 
@@ -268,7 +263,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
             cb = rd.cell_base;
             row = grow - rd.row_base;
             if (R != cur_r) {
-                flush();
+                if constexpr (W) flush();
                 if (cur_r >= 0)
                     drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + (size_t)cur_r * nbt,
                           a.out_w + (size_t)cur_r * 2 * nbt);
@@ -321,15 +316,14 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                 const int nt = min(kWave, je - jt);
                 const int jmin = max(self ? i + 1 - jt : 0, 0);   // self row: only j > i
                 if (vi) st.cand += (unsigned long long)(nt - min(jmin, nt));
-                int nup = 0;    // LDS bin increments of this lane in this tile
+                bound += (unsigned long long)nt * 2ull * kWave;
                 // histogram update of one kept pair with neighbour flags fj, bin b, mu bins m / mr
                 auto finish = [&](uint32_t fj, double wj, int b, int m, int mr) {
                     const int cj = (fj & 2) ? 0 : 1;
                     const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
                     double x = 0.0;
                     if constexpr (W) x = wi * wj;
-                    nup += accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt,
-                                              a.out_w);
+                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt, a.out_w);
                 };
                 if constexpr (FP64) {
                     wave_sync();
@@ -418,13 +412,11 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 if constexpr (W) quantise(wi * ewj, qlo, qhi);
                                 if constexpr (W && !RSD) {
                                     if (ci + cj) put(base, (uint32_t)(ci + cj), qlo, qhi);
-                                    nup += ci + cj;
                                 } else if (W && mirror && mr == nm - 1 - m) {
                                     put(base + m, 1u, qlo, qhi);   // stands for i->j at m and j->i at nm-1-m
-                                    ++nup;
                                 } else {
-                                    nup += accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj,
-                                                              wi * ewj, a.out_cnt, a.out_w);
+                                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, wi * ewj,
+                                                       a.out_cnt, a.out_w);
                                 }
                             }
                         } while (false);
@@ -435,11 +427,10 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         eidx = rot_lane(eidx);
                     }
                 }
-                // each wave drains after drain_at of its own increments (the increments held in the
-                // lanes' run caches included), which bounds every LDS sum
-                bound += (unsigned long long)wave_sum(nup);
+                // each wave drains after drain_at of its own possible increments (every candidate
+                // counted twice; the increments held in the lanes' run caches included)
                 if (bound > drain_at) {
-                    flush();
+                    if constexpr (W) flush();
                     const size_t ro = batched ? (size_t)cur_r : 0;
                     drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + ro * nbt,
                           a.out_w + ro * 2 * nbt);
@@ -448,7 +439,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
             }
         }
     }
-    flush();
+    if constexpr (W) flush();
     if (batched) {
         if (cur_r >= 0)
             drain(hc, hw, nbt, nm, reps, W, mirror, lane, kWave, a.out_cnt + (size_t)cur_r * nbt,

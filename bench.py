@@ -32,6 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "binned pair/triplet evals/s on 1 MI355X: batched FP32+guard vs plain FP64 vs CPU"
+# HBM bytes per dispatch of the FP32 + guard pair kernel on the (r, mu) workload, measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) and scaled by the same counters'
+# readings for a copy of known size (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE)
+PAIR_RMU_TRAFFIC_BYTES = (36_368_985.9 * 2.0 + 132_062.3 * 1.0) * 1024
 FLOPS_PER_PAIR_RMU = 22      # 3 sub, 3 mul, 2 add for s; 3 mid adds, m.s (5), |m|^2 (5), 1 product
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector, dense
 PEAK_FP64 = 78.6e12
@@ -74,19 +78,20 @@ def time_gpu(eng, dxyz, dw, nd, precision, steps, warmup):
 def cpu_baseline(xyz, w, nd):
     """The reference program (oracle/_ref/gramsci_fast: the reference sources
     built with its own -O3 -ffast-math optimisation) on a density-matched
-    100^3 sub-box of the same catalogue, same bins, all host threads used.
+    150^3 sub-box of the same catalogue, same bins, all host threads used.
     Throughput = binned ordered pairs / (graph + query time printed by the
     reference).  The reference's cost per pair grows with the neighbour count
-    k (per-point selection sort), and k is ~15x smaller in the sub-box than in
-    the full box, so this over-states its throughput on the full workload."""
+    k (per-point selection sort), and k is several times smaller in the
+    sub-box than in the full box (rmax = 150 is the sub-box side), so this
+    over-states its throughput on the full workload."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ref = O.REF_BIN_FAST if os.path.exists(O.REF_BIN_FAST) else O.REF_BIN
     if not os.path.exists(ref):
         return None
-    side = 100.0
-    lo = 450.0
+    side = 150.0
+    lo = 425.0
     inb = np.all((xyz >= lo) & (xyz < lo + side), axis=1)
     idx = np.nonzero(inb)[0]
     d_idx = idx[idx < nd]
@@ -217,8 +222,8 @@ def bench_rmu(args):
                      n_points=int(len(xyz)), binned_pairs_per_step=int(U),
                      precision="FP32 + FP64 guard-band recheck" if main_prec == "fp32" else "plain FP64")
     line["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                        "frac": achieved / peak, "traffic": None,
-                        "kernel": f"k_pair2<{'true' if main_prec == 'fp64' else 'false'},true,true>",
+                        "frac": achieved / peak, "traffic": PAIR_RMU_TRAFFIC_BYTES if main_prec == "fp32" else None,
+                        "kernel": f"k_pair2<{'true' if main_prec == 'fp64' else 'false'}, true, true, true>",
                         "kernel_ms": R["t_kern"], "flops_per_pair": FLOPS_PER_PAIR_RMU}
     if "fp64" in res and main_prec != "fp64":
         F = res["fp64"]

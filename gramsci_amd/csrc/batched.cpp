@@ -38,6 +38,7 @@ void batch_release(GsBatch& b)
     b.d_off.release(); b.d_row_base.release(); b.cell_of.release(); b.count.release(); b.cell_start.release();
     b.nch.release(); b.row_chunk.release(); b.scan_tmp.release(); b.queue.release(); b.orig.release();
     b.d_regions.release(); b.params.release(); b.pw.release(); b.pl.release(); b.flags.release(); b.out.release();
+    b.xyz.release(); b.w.release(); b.buf.release();
     b.valid = false;
 }
 
@@ -67,9 +68,9 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
             return GS_OK;
         }
     const int nreg = B.nreg;
-    const int64_t n = v.n_data;   // every point of the batch is held as "data" in the context buffers
+    const int64_t n = B.off[nreg];   // the batch's own copy of its catalogue (input order)
     const int nb = p.nbins, nm = p.nmu;
-    const bool rsd = nm >= 2, fp64 = p.precision == GS_FP64, weighted = v.has_w;
+    const bool rsd = nm >= 2, fp64 = p.precision == GS_FP64, weighted = B.has_w;
     const int nbt = 3 * nb * nm;
 
     // cell edge and per-region grids
@@ -183,7 +184,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     a.g.m = mst;
     a.nrows = (int)rows.size();
     a.reps = 1;
-    a.mirror = (rsd && !v.has_buf) ? 1 : 0;
+    a.mirror = (rsd && !B.has_buf) ? 1 : 0;
     a.regions = d_reg;
     a.row_base = (const int32_t*)(pblob->ptr + rb_off);
     a.n_chunks_dev = B.row_chunk.ptr + rows_tot;
@@ -226,11 +227,12 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 16, 0, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.out.ptr, (int64_t)nout * 2, 0, v.stream);
     if (e == hipSuccess)
-        e = launch_cell_count_b(v.xyz, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
+        e = launch_cell_count_b(B.xyz.ptr, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
     if (e == hipSuccess) e = exclusive_scan(B.count.ptr, B.cell_start.ptr, cells + 1, B.scan_tmp.ptr, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
     if (e == hipSuccess)
-        e = launch_scatter_b(v.xyz, v.w, v.buf, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.cell_start.ptr,
+        e = launch_scatter_b(B.xyz.ptr, B.has_w ? B.w.ptr : nullptr, B.has_buf ? B.buf.ptr : nullptr, n, B.d_off.ptr,
+                             nreg, d_reg, h, B.cell_of.ptr, B.cell_start.ptr,
                              B.count.ptr, B.pw.ptr, B.pl.ptr, B.flags.ptr, B.orig.ptr, v.stream);
     if (kind == 2) {
         if (e == hipSuccess)
@@ -322,7 +324,17 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
         }
     }
     if (w) B.wmax = wm > 0 ? wm : 1.0;
-    if ((rc = ctx_upload(c, xyz, w, buf, n))) return rc;
+    // the batch keeps its own copy of the catalogue, so that later single-catalogue
+    // uploads on this context cannot change what the captured pipelines read
+    B.has_w = w != nullptr;
+    B.has_buf = buf != nullptr;
+    if ((rc = B.xyz.reserve(3 * std::max<int64_t>(n, 1))) || (B.has_w && (rc = B.w.reserve(std::max<int64_t>(n, 1)))) ||
+        (B.has_buf && (rc = B.buf.reserve(std::max<int64_t>(n, 1)))))
+        return rc;
+    hipError_t eu = n > 0 ? hipMemcpy(B.xyz.ptr, xyz, (size_t)n * 24, hipMemcpyHostToDevice) : hipSuccess;
+    if (eu == hipSuccess && B.has_w && n > 0) eu = hipMemcpy(B.w.ptr, w, (size_t)n * 8, hipMemcpyHostToDevice);
+    if (eu == hipSuccess && B.has_buf && n > 0) eu = hipMemcpy(B.buf.ptr, buf, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (eu != hipSuccess) return hip_fail(eu, "batched catalogue upload");
     std::vector<int32_t> o32(nreg + 1);
     for (int r = 0; r <= nreg; ++r) o32[r] = (int32_t)off[r];
     if ((rc = B.d_off.reserve(nreg + 1))) return rc;
@@ -343,7 +355,7 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
     GsBatch& B = *v.batch;
     if (!B.valid) return fail(GS_EINVAL, "no batched catalogue (gs_set_catalogue_batched)");
     const int nreg = B.nreg, nb = p->nbins, nm = p->nmu, nbt = 3 * nb * nm;
-    const bool weighted = v.has_w;
+    const bool weighted = B.has_w;
 
     int m = p->cell_div;
     if (m == 0) {
@@ -420,7 +432,7 @@ extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
     if (!B.valid) return fail(GS_EINVAL, "no batched catalogue (gs_set_catalogue_batched)");
     const int nreg = B.nreg, nb = p->nbins, nm = p->nmu;
     const int ncfg = nb * (nb + 1) * (nb + 2) / 6;
-    const bool weighted = v.has_w;
+    const bool weighted = B.has_w;
 
     int m = p->cell_div;
     if (m == 0) {

@@ -48,6 +48,9 @@ struct GsBatch {
     DevBuf<float4> pl;
     DevBuf<uint32_t> flags;
     DevBuf<unsigned long long> out;
+    DevBuf<double> xyz, w;               // the batch's catalogue, input order
+    DevBuf<int32_t> buf;
+    bool has_w = false, has_buf = false;
     struct Plan {
         gs_params p{};
         int m = 0;

@@ -104,3 +104,23 @@ def test_batched_3pcf_equals_separate_launches_and_replays(eng, batch):
     for r in range(len(ndata)):
         assert np.array_equal(a["cnt"][r][:, 0], sep[r]["cnt"]), r
         assert np.array_equal(a["wsum"][r][:, 0], sep[r]["wsum"]), r
+
+
+def test_batch_survives_single_catalogue_uploads(eng, batch):
+    # the batch keeps its own copy of the catalogue: uploading a single catalogue on the
+    # same context (smaller or larger) changes neither replays of captured pipelines nor
+    # pipelines captured afterwards
+    xyz, buf, off, ndata = batch
+    eng.set_catalogue_batched(xyz, None, buf, off, ndata)
+    a2 = eng.pcf2_batched(1.0, 15.0, 6, 3, False, "fp32", cell_div=3)
+    small = C.uniform(500, 30.0, 5)
+    eng.set_catalogue(small, None, len(small))
+    b2 = eng.pcf2_batched(1.0, 15.0, 6, 3, False, "fp32", cell_div=3)        # replay
+    big = C.uniform(len(xyz) * 2, 400.0, 6)
+    eng.set_catalogue(big, None, len(big))
+    a3 = eng.pcf3_batched(2.0, 8.0, 4, 1, False, "fp32", cell_div=2)          # new capture
+    assert np.array_equal(a2["cnt"], b2["cnt"])
+    for r in range(len(ndata)):
+        sl = slice(off[r], off[r + 1])
+        ref = O.pcf3(xyz[sl], None, int(ndata[r]), O.Params(2.0, 8.0, 4), buf=buf[sl])
+        assert np.array_equal(a3["cnt"][r][:, 0], ref["cnt"]), r

@@ -52,18 +52,25 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
 }
 
 // record packing: bits 0..7 bin (0-based), bit 8 random, bits 9..15 mu bin (0-based)
-template <bool FP64, bool W>
+//
+// The variant is a template parameter so that each one pays only for its own
+// registers: MODE bit 0 RSD (nm >= 2), bit 1 equilateral (0 = plain); GEN 1 for
+// a batched launch or a histogram in global memory (the plain launch with an
+// LDS histogram, the common case, runs with GEN 0).
+enum { kT3Rsd = 1, kT3Equi = 2 };
+
+template <bool FP64, bool W, int MODE, bool GEN>
 __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const BinDesc& B = *a.bins;
     const int nb = B.nb;
-    const int nm = a.nm;
+    const int nm = (MODE & kT3Rsd) ? a.nm : 1;
     const int nh = a.ncfg * nm * 4;
     const int reps = a.reps;
     const int kmax = a.kmax;
-    const bool lh = a.lds_hist != 0;
-    const bool equi = a.equi != 0;
+    const bool lh = !GEN || a.lds_hist != 0;
+    constexpr bool equi = (MODE & kT3Equi) != 0;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int rep = lane & (reps - 1);
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
     const int shift = B.fx_shift3;
     const double inv_h = 1.0 / a.g.h;
-    const bool batched = a.nreg > 0;
+    const bool batched = GEN && a.nreg > 0;
     int cur_r = batched ? -1 : 0;   // region whose histogram the LDS holds (work-group uniform)
 
     // adds the LDS histogram to region r's output and clears it (all threads of the work-group)
@@ -347,10 +354,30 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     }
 }
 
-template <bool FP64, bool W>
-static hipError_t launch_t3(const TripArgs& a, size_t lds, int nblocks, hipStream_t st)
+using T3Kernel = void (*)(TripArgs);
+
+template <bool FP64, bool W, int MODE>
+static T3Kernel pick_t3m(bool gen)
 {
-    auto kern = k_triplet3<FP64, W>;
+    return gen ? k_triplet3<FP64, W, MODE, true> : k_triplet3<FP64, W, MODE, false>;
+}
+
+template <bool FP64, bool W>
+static T3Kernel pick_t3w(const TripArgs& a)
+{
+    const bool gen = a.nreg > 0 || !a.lds_hist;
+    if (a.equi) return a.nm > 1 ? pick_t3m<FP64, W, kT3Equi | kT3Rsd>(gen) : pick_t3m<FP64, W, kT3Equi>(gen);
+    return a.nm > 1 ? pick_t3m<FP64, W, kT3Rsd>(gen) : pick_t3m<FP64, W, 0>(gen);
+}
+
+static T3Kernel pick_t3(const TripArgs& a, bool fp64, bool weighted)
+{
+    if (fp64) return weighted ? pick_t3w<true, true>(a) : pick_t3w<true, false>(a);
+    return weighted ? pick_t3w<false, true>(a) : pick_t3w<false, false>(a);
+}
+
+static hipError_t launch_t3(T3Kernel kern, const TripArgs& a, size_t lds, int nblocks, hipStream_t st)
+{
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -361,24 +388,19 @@ static hipError_t launch_t3(const TripArgs& a, size_t lds, int nblocks, hipStrea
 
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st)
 {
-    const size_t lds = triplet3_lds_bytes(a, nb, fp64, weighted);
-    if (fp64) return weighted ? launch_t3<true, true>(a, lds, nblocks, st) : launch_t3<true, false>(a, lds, nblocks, st);
-    return weighted ? launch_t3<false, true>(a, lds, nblocks, st) : launch_t3<false, false>(a, lds, nblocks, st);
-}
-
-template <bool FP64, bool W>
-static int occ3(size_t lds)
-{
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_triplet3<FP64, W>, kWG, lds) != hipSuccess) return 1;
-    return nb > 0 ? nb : 1;
+    return launch_t3(pick_t3(a, fp64, weighted), a, triplet3_lds_bytes(a, nb, fp64, weighted), nblocks, st);
 }
 
 int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted)
 {
     const size_t lds = triplet3_lds_bytes(a, nb, fp64, weighted);
-    if (fp64) return weighted ? occ3<true, true>(lds) : occ3<true, false>(lds);
-    return weighted ? occ3<false, true>(lds) : occ3<false, false>(lds);
+    T3Kernel kern = pick_t3(a, fp64, weighted);
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 1;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kWG, lds) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
 
 }  // namespace gs

@@ -148,6 +148,7 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool 
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
     const int waves = wg / kWave;
     if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // LDS tile: x, y, z, w, flags, pad
+    else if (!weighted) b += (size_t)waves * kWave * (16 + 4);  // LDS tile: float4, flags
     return b;
 }
 
@@ -182,8 +183,11 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
-    // FP64: wave-private LDS tiles x[64], y[64], z[64], w[64], flags[64]
-    unsigned char* tb = p + (size_t)wv * kWave * 40;
+    // wave-private LDS tiles: FP64 x[64], y[64], z[64], w[64], flags[64];
+    // FP32 unweighted float4[64], flags[64]
+    unsigned char* tb = p + (size_t)wv * kWave * (FP64 ? 40 : 20);
+    float4* tl = (float4*)tb;
+    uint32_t* tlf = (uint32_t*)(tl + kWave);
     double* tx = (double*)tb;
     double* ty = tx + kWave;
     double* tz = ty + kWave;
@@ -354,24 +358,89 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         finish(fj, W ? tw[kk] : 0.0, b, m, mr);
                     }
                 } else {
-                    // Register tile: lane L loads entry L; the entries then rotate through the
-                    // wave one lane per step (DPP), so after 64 steps every lane has met every
-                    // entry.  Consecutive lanes meet different neighbours at each step, which
-                    // spreads their histogram updates over many bins, and no LDS is read for
-                    // the neighbours.
-                    const bool has = lane < nt;
-                    const int jl = jt + (has ? lane : 0);
-                    float4 e = a.pl[jl];
-                    double ewj = 0.0;
-                    if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
-                    uint32_t ef = a.flags[jl];
-                    int eidx = has ? lane : -1;
-                    for (int k = 0; k < kWave; ++k) {
-                        if (vi && eidx >= jmin) do {
+                    if constexpr (W) {
+                        // Register tile: lane L loads entry L; the entries then rotate through the
+                        // wave one lane per step (DPP), so after 64 steps every lane has met every
+                        // entry.  Consecutive lanes meet different neighbours at each step, which
+                        // spreads their histogram updates over many bins, and no LDS is read for
+                        // the neighbours.  Used for weighted sums, where it beats the LDS tile.
+                        const bool has = lane < nt;
+                        const int jl = jt + (has ? lane : 0);
+                        float4 e = a.pl[jl];
+                        double ewj = 0.0;
+                        if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
+                        uint32_t ef = a.flags[jl];
+                        int eidx = has ? lane : -1;
+                        for (int k = 0; k < kWave; ++k) {
+                            if (vi && eidx >= jmin) do {
+                                const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
+                                const float dy = e.y - yi_r, dz = e.z - zi_r;
+                                const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                                if (!(s > rej_lo && s < rej_hi)) break;
+                                int b = (int)bin_estimate(s, LOGB, ua, ub);
+                                b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                                const float2 ok = okb[b];
+                                bool safe = (s >= ok.x) && (s < ok.y);
+                                int m = 0, mr = 0;
+                                if constexpr (RSD) {
+                                    const float r = __builtin_amdgcn_rsqf(s);
+                                    // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
+                                    const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
+                                    const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
+                                    const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
+                                    const float rm = __builtin_amdgcn_rsqf(M2);
+                                    const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
+                                    const float dt = ms32 * fmaf(mu_ew, r, fmaf(mu_em, rm, mu_c));
+                                    // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
+                                    // nearest integer is no edge or is farther than the error bound dt < 1/2
+                                    const float rt = rintf(t);
+                                    const bool edge = (rt >= 1.0f) && (rt <= (float)(nm - 1));
+                                    safe = safe && (dt < 0.5f) && (!edge || fabsf(t - rt) >= dt);
+                                    const int mi = (int)floorf(t);
+                                    m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
+                                    mr = nm - 1 - m;
+                                }
+                                if (!safe) {
+                                    ++st.rechk;
+                                    const double4 Pj = a.pw[jt + eidx];
+                                    Pi = a.pw[i];
+                                    const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
+                                    if (!(se > t_lo && se < t_hi)) break;
+                                    b = exact_bin(se, t64, nb);
+                                    if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); break; }
+                                    if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                }
+                                {
+                                    const int cj = (ef & 2) ? 0 : 1;
+                                    const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
+                                    uint32_t qlo = 0, qhi = 0;
+                                    if constexpr (W) quantise(wi * ewj, qlo, qhi);
+                                    if constexpr (W && !RSD) {
+                                        if (ci + cj) put(base, (uint32_t)(ci + cj), qlo, qhi);
+                                    } else if (W && mirror && mr == nm - 1 - m) {
+                                        put(base + m, 1u, qlo, qhi);   // stands for i->j at m and j->i at nm-1-m
+                                    } else {
+                                        accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, wi * ewj,
+                                                           a.out_cnt, a.out_w);
+                                    }
+                                }
+                            } while (false);
+                            // rotate the register tile by one lane (all lanes active here)
+                            e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
+                            if constexpr (W) ewj = rot_lane(ewj);
+                            ef = rot_lane(ef);
+                            eidx = rot_lane(eidx);
+                        }
+                    } else {
+                        // Unweighted: LDS tile with the rotated schedule of the FP64 path (a partial
+                        // tile costs only its nt steps; faster than the register tile when no weight
+                        // product rides along).  eval32 bins the pair of this lane's point and tile
+                        // entry eidx, with the guard band and the FP64 recheck.
+                        auto eval32 = [&](const float4 e, const uint32_t ef, const int eidx) {
                             const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
                             const float dy = e.y - yi_r, dz = e.z - zi_r;
                             const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                            if (!(s > rej_lo && s < rej_hi)) break;
+                            if (!(s > rej_lo && s < rej_hi)) return;
                             int b = (int)bin_estimate(s, LOGB, ua, ub);
                             b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
                             const float2 ok = okb[b];
@@ -400,31 +469,30 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const double4 Pj = a.pw[jt + eidx];
                                 Pi = a.pw[i];
                                 const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
-                                if (!(se > t_lo && se < t_hi)) break;
+                                if (!(se > t_lo && se < t_hi)) return;
                                 b = exact_bin(se, t64, nb);
-                                if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); break; }
+                                if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); return; }
                                 if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
                             }
-                            {
-                                const int cj = (ef & 2) ? 0 : 1;
-                                const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
-                                uint32_t qlo = 0, qhi = 0;
-                                if constexpr (W) quantise(wi * ewj, qlo, qhi);
-                                if constexpr (W && !RSD) {
-                                    if (ci + cj) put(base, (uint32_t)(ci + cj), qlo, qhi);
-                                } else if (W && mirror && mr == nm - 1 - m) {
-                                    put(base + m, 1u, qlo, qhi);   // stands for i->j at m and j->i at nm-1-m
-                                } else {
-                                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, wi * ewj,
-                                                       a.out_cnt, a.out_w);
-                                }
-                            }
-                        } while (false);
-                        // rotate the register tile by one lane (all lanes active here)
-                        e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
-                        if constexpr (W) ewj = rot_lane(ewj);
-                        ef = rot_lane(ef);
-                        eidx = rot_lane(eidx);
+                            const int cj = (ef & 2) ? 0 : 1;
+                            const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
+                            accumulate<RSD, false>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, 0.0, a.out_cnt,
+                                                   a.out_w);
+                        };
+                        wave_sync();
+                        if (lane < nt) {
+                            tl[lane] = a.pl[jt + lane];
+                            tlf[lane] = a.flags[jt + lane];
+                        }
+                        wave_sync();
+                        const int kend = vi ? nt : 0;
+                        const int lm = lane % nt;
+                        for (int k = 0; k < kend; ++k) {
+                            int kk = k + lm;
+                            kk = kk >= nt ? kk - nt : kk;
+                            if (kk < jmin) continue;
+                            eval32(tl[kk], tlf[kk], kk);
+                        }
                     }
                 }
                 // each wave drains after drain_at of its own possible increments (every candidate

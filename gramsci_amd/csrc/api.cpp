@@ -327,6 +327,7 @@ struct gs_ctx {
     DevBuf<unsigned long long> out;
     DevBuf<int32_t> queue;
     DevBuf<double> part;
+    DevBuf<unsigned long long> wpart;
     GsExtra extra;
     GsBatch batch;
 };
@@ -379,7 +380,7 @@ extern "C" int gs_destroy(gs_ctx* c)
     c->cell_of.release(); c->cell_start.release(); c->fill.release(); c->chunk_nc.release();
     c->chunk_off.release(); c->scan_tmp.release(); c->pw.release(); c->pl.release(); c->flags.release();
     c->orig.release();
-    c->params.release(); c->out.release(); c->queue.release(); c->part.release();
+    c->params.release(); c->out.release(); c->queue.release(); c->part.release(); c->wpart.release();
     extra_release(c->extra);
     batch_release(c->batch);
     for (auto& ev : c->ev)
@@ -416,6 +417,17 @@ static int finish_catalogue(gs_ctx* c)
     return GS_OK;
 }
 
+long double gs::fixed_sum(const double* w, int64_t n, int k)
+{
+    unsigned __int128 s = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t lo, hi;
+        wsum_fixed(w[i], k, lo, hi);
+        s += ((unsigned __int128)hi << 64) | lo;
+    }
+    return std::ldexp((long double)s, -k);
+}
+
 static int host_weight_stats(gs_ctx* c, const double* w)
 {
     c->wsum_d = 0;
@@ -431,8 +443,51 @@ static int host_weight_stats(gs_ctx* c, const double* w)
     for (int64_t i = 0; i < n; ++i) {
         if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return fail(GS_EINVAL, "weights must be finite and >= 0 (w[%lld]=%g)", (long long)i, w[i]);
         wm = std::max(wm, w[i]);
-        if (i < c->n_data) c->wsum_d += w[i]; else c->wsum_r += w[i];
     }
+    const int k = wsum_scale(wm);
+    c->wsum_d = fixed_sum(w, c->n_data, k);
+    c->wsum_r = fixed_sum(w + c->n_data, c->n_rand, k);
+    c->wmax = wm > 0 ? wm : 1.0;
+    return GS_OK;
+}
+
+// The same statistics for weights already on the device (c->w): the check, the maximum and the
+// fixed-point sums run as reductions on the GPU; only per-block partials come back.
+static int device_weight_stats(gs_ctx* c)
+{
+    const int64_t n = c->n_data + c->n_rand;
+    const int nblk = 256;
+    int rc;
+    if ((rc = c->part.reserve(nblk * 6)) || (rc = c->wpart.reserve(nblk * 4 + 1))) return rc;
+    unsigned long long* bad = c->wpart.ptr + nblk * 4;
+    hipError_t e = launch_fill_u32(bad, 2, 0xffffffffu, c->stream);
+    if (e == hipSuccess) e = launch_wcheck(c->w.ptr, n, c->part.ptr, bad, nblk, c->stream);
+    std::vector<double> pmax(nblk);
+    unsigned long long first_bad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(pmax.data(), c->part.ptr, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&first_bad, bad, sizeof(first_bad), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "weight check");
+    if (first_bad != ~0ull) {
+        double v = 0.0;
+        (void)hipMemcpy(&v, c->w.ptr + first_bad, sizeof(double), hipMemcpyDeviceToHost);
+        return fail(GS_EINVAL, "weights must be finite and >= 0 (w[%llu]=%g)", first_bad, v);
+    }
+    double wm = 0.0;
+    for (double x : pmax) wm = std::max(wm, x);
+    const int k = wsum_scale(wm);
+    std::vector<unsigned long long> ps(nblk * 4);
+    e = launch_wsum(c->w.ptr, n, c->n_data, k, c->wpart.ptr, nblk, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ps.data(), c->wpart.ptr, ps.size() * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return hip_fail(e, "weight sums");
+    unsigned __int128 sd = 0, sr = 0;
+    for (int b = 0; b < nblk; ++b) {
+        sd += ((unsigned __int128)ps[4 * b + 1] << 64) | ps[4 * b];
+        sr += ((unsigned __int128)ps[4 * b + 3] << 64) | ps[4 * b + 2];
+    }
+    c->wsum_d = std::ldexp((long double)sd, -k);
+    c->wsum_r = std::ldexp((long double)sr, -k);
     c->wmax = wm > 0 ? wm : 1.0;
     return GS_OK;
 }
@@ -480,20 +535,20 @@ extern "C" int gs_set_catalogue_device(gs_ctx* c, const double* d_xyz, const dou
     c->has_buf = d_buf != nullptr;
     if ((rc = c->xyz.reserve(3 * n))) return rc;
     hipError_t e = hipMemcpyAsync(c->xyz.ptr, d_xyz, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
-    std::vector<double> hw;
     if (e == hipSuccess && d_w) {
         if ((rc = c->w.reserve(n))) return rc;
         e = hipMemcpyAsync(c->w.ptr, d_w, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
-        hw.resize(n);
-        if (e == hipSuccess) e = hipMemcpyAsync(hw.data(), d_w, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
     if (e == hipSuccess && d_buf) {
         if ((rc = c->buf.reserve(n))) return rc;
         e = hipMemcpyAsync(c->buf.ptr, d_buf, n * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream);
     }
     if (e != hipSuccess) return hip_fail(e, "catalogue copy");
-    if ((rc = host_weight_stats(c, d_w ? hw.data() : nullptr))) return rc;
+    if ((rc = d_w && n > 0 ? device_weight_stats(c) : host_weight_stats(c, nullptr))) {
+        c->n_data = c->n_rand = 0;      // no half-set catalogue stays behind
+        c->grid_valid = false;
+        return rc;
+    }
     return finish_catalogue(c);
 }
 

@@ -309,6 +309,7 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
         for (int k = 0; k < 3; ++k) { bb[k] = 1e308; bb[3 + k] = -1e308; }
         if (off[r + 1] == off[r])
             for (int k = 0; k < 6; ++k) bb[k] = 0.0;
+        double wmr = 0.0;
         for (int64_t i = off[r]; i < off[r + 1]; ++i) {
             for (int k = 0; k < 3; ++k) {
                 const double x = xyz[3 * i + k];
@@ -320,7 +321,16 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
             const double wi = w ? w[i] : 1.0;
             if (!(wi >= 0.0) || !std::isfinite(wi)) return fail(GS_EINVAL, "weights must be finite and >= 0");
             wm = std::max(wm, wi);
-            if (i - off[r] < ndata[r]) B.wd[r] += wi; else B.wr[r] += wi;
+            wmr = std::max(wmr, wi);
+        }
+        // the region's sums exactly as a single-catalogue upload of the region computes them
+        if (w) {
+            const int k = wsum_scale(wmr);
+            B.wd[r] = fixed_sum(w + off[r], ndata[r], k);
+            B.wr[r] = fixed_sum(w + off[r] + ndata[r], off[r + 1] - off[r] - ndata[r], k);
+        } else {
+            B.wd[r] = (long double)ndata[r];
+            B.wr[r] = (long double)(off[r + 1] - off[r] - ndata[r]);
         }
     }
     if (w) B.wmax = wm > 0 ? wm : 1.0;

@@ -34,6 +34,59 @@ __global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ xyz, in
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ---------------------------------------------------------------- weights
+__global__ __launch_bounds__(256) void k_wcheck(const double* __restrict__ w, int64_t n, double* __restrict__ part_max,
+                                                unsigned long long* __restrict__ bad)
+{
+    __shared__ double red[256];
+    double mx = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = w[i];
+        if (!(v >= 0.0) || !isfinite(v)) atomicMin(bad, (unsigned long long)i);
+        else mx = fmax(mx, v);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part_max[blockIdx.x] = red[0];
+}
+
+__device__ __forceinline__ void add128(unsigned long long& lo, unsigned long long& hi, unsigned long long l,
+                                       unsigned long long h)
+{
+    lo += l;
+    hi += h + (lo < l ? 1ull : 0ull);
+}
+
+__global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ w, int64_t n, int64_t n_data, int k,
+                                              unsigned long long* __restrict__ part)
+{
+    __shared__ unsigned long long red[4][256];
+    unsigned long long a[4] = {0, 0, 0, 0};     // data lo, hi, randoms lo, hi
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t lo, hi;
+        wsum_fixed(w[i], k, lo, hi);
+        const int o = i < n_data ? 0 : 2;
+        add128(a[o], a[o + 1], lo, hi);
+    }
+    for (int q = 0; q < 4; ++q) red[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const int t = threadIdx.x;
+            unsigned long long l0 = red[0][t], h0 = red[1][t], l1 = red[2][t], h1 = red[3][t];
+            add128(l0, h0, red[0][t + s], red[1][t + s]);
+            add128(l1, h1, red[2][t + s], red[3][t + s]);
+            red[0][t] = l0; red[1][t] = h0; red[2][t] = l1; red[3][t] = h1;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // ---------------------------------------------------------------- cell of point
 __device__ __forceinline__ int cell_coord(double v, double lo, double inv_h, int dim)
 {
@@ -187,6 +240,20 @@ hipError_t launch_row_chunks(const int32_t* start, int nx, int nrow, int32_t* nc
 hipError_t launch_bbox(const double* xyz, int64_t n, double* part, int nblk, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bbox, dim3(nblk), dim3(256), 0, st, xyz, n, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_wcheck(const double* w, int64_t n, double* part_max, unsigned long long* bad, int nblk,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wcheck, dim3(nblk), dim3(256), 0, st, w, n, part_max, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_wsum(const double* w, int64_t n, int64_t n_data, int k, unsigned long long* part, int nblk,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wsum, dim3(nblk), dim3(256), 0, st, w, n, n_data, k, part);
     return hipGetLastError();
 }
 

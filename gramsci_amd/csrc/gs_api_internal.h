@@ -100,6 +100,8 @@ double round11(double h);
 int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
 int check_params(const gs_params* p, int mode);
 int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B);
+// sum of w[0..n) as the exact sum of the fixed-point integers wsum_fixed(w[i], k), as a long double
+long double fixed_sum(const double* w, int64_t n, int k);
 std::vector<Row> half_stencil(double h, double rmax, int m);
 std::vector<Row> full_stencil(double h, double rmax, int m);
 double ctx_xmax(gs_ctx* c);

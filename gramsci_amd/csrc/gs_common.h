@@ -56,6 +56,27 @@ struct RegionDesc {
     int32_t pad;
 };
 
+// Exact, order-independent weight sums (the normalisations sum w over data
+// and over randoms, gramsci.F90:1390-1391).  Each weight is rounded to the
+// integer q = rint(w * 2^k) < 2^95, with k = wsum_scale(wmax); a sum of up to
+// 2^31 such integers fits in 128 bits, so the host loops and the device
+// reduction give identical sums in any order.  Relative to wmax, a weight
+// loses at most 2^-96 in the rounding.
+inline int wsum_scale(double wmax)
+{
+    int e = 0;
+    if (wmax > 0) (void)frexp(wmax, &e);   // wmax < 2^e
+    return 95 - e;
+}
+
+__host__ __device__ inline void wsum_fixed(double w, int k, uint64_t& lo, uint64_t& hi)
+{
+    const double x = ldexp(w, k);                   // < 2^95, exact
+    const double h = floor(ldexp(x, -64));          // < 2^31
+    lo = (uint64_t)rint(x - ldexp(h, 64));          // the difference is exact and < 2^64
+    hi = (uint64_t)h;
+}
+
 // Half-stencil rows of the pair kernel: cells (ax+dx, ay+dy, az+dz) with dx in [x0, x1].
 struct Row {
     int8_t dy, dz, x0, x1;

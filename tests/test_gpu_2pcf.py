@@ -151,6 +151,38 @@ def test_device_pointer_input(eng):
     r = eng.pcf2(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32")
     assert np.array_equal(r["cnt"], ref["cnt"])
     assert _close(r["n2"], ref["n2"]) and _close(r["n3"], ref["n3"])
+    # the weight sums of a device upload (GPU reduction) and of a host upload (host loop) are the
+    # same exact fixed-point sum, so the normalised results agree to the last bit
+    eng.set_catalogue(xyz, w, 5000)
+    h = eng.pcf2(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32")
+    assert np.array_equal(h["n2"], r["n2"]) and np.array_equal(h["n3"], r["n3"])
+
+
+def test_device_weights_wide_range_and_errors(eng):
+    import torch
+    d = C.clustered(4000, 100.0, 16)
+    r = C.uniform(8000, 100.0, 17)
+    xyz = np.concatenate([d, r])
+    rng = np.random.default_rng(18)
+    w = 10.0 ** rng.uniform(-12, 6, len(xyz))        # 18 decades of weights
+    w[::97] = 0.0
+    p = O.Params(2.0, 20.0, 8, 3, False)
+    ref = O.pcf2(xyz, w, 4000, p)
+    eng.set_catalogue(torch.from_numpy(xyz).cuda(), torch.from_numpy(w).cuda(), 4000)
+    a = eng.pcf2(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32")
+    assert np.array_equal(a["cnt"], ref["cnt"])
+    assert _close(a["n2"], ref["n2"]) and _close(a["n3"], ref["n3"])
+    eng.set_catalogue(xyz, w, 4000)
+    b = eng.pcf2(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32")
+    assert np.array_equal(a["n2"], b["n2"])
+    for bad in (np.nan, -1.0, np.inf):
+        wb = w.copy()
+        wb[1234] = bad
+        with pytest.raises(Exception, match=r"w\[1234\]"):
+            eng.set_catalogue(torch.from_numpy(xyz).cuda(), torch.from_numpy(wb).cuda(), 4000)
+    eng.set_catalogue(torch.from_numpy(xyz).cuda(), torch.from_numpy(w).cuda(), 4000)   # usable again
+    c = eng.pcf2(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32")
+    assert np.array_equal(a["n2"], c["n2"])
 
 
 def test_fp32_equals_fp64_large(eng):

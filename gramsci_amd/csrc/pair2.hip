@@ -297,6 +297,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
             fi = a.flags[i];
         }
         const int ri = fi & 1, ci = (fi & 2) ? 0 : 1;
+        // a chunk of buffer points only (never apexes) gains nothing from a tile of buffer points only
+        const bool chunk_buf = __ballot(ci != 0) == 0;
         const double wi = Pi.w;
         const float ax32 = (float)Pi.x, ay32 = (float)Pi.y, az32 = (float)Pi.z;   // absolute, for mu
         // histogram class offsets of this lane's pairs with a data / a random neighbour
@@ -331,17 +333,20 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                 };
                 if constexpr (FP64) {
                     wave_sync();
+                    uint32_t fl = 2u;
                     if (lane < nt) {
                         const double4 Pj = a.pw[jt + lane];
                         tx[lane] = Pj.x; ty[lane] = Pj.y; tz[lane] = Pj.z;
                         if (W) tw[lane] = ldexp(Pj.w, shift);
-                        tf[lane] = a.flags[jt + lane];
+                        fl = a.flags[jt + lane];
+                        tf[lane] = fl;
                     }
+                    const bool skip = chunk_buf && __ballot((fl & 2u) == 0) == 0;
                     wave_sync();
                     // rotated schedule: at step k lane L takes tile entry (k + L) mod nt, so the
                     // lanes of a wave pair with different neighbours (spreading their histogram
                     // updates over many bins) while reading consecutive, conflict-free LDS words
-                    const int kend = vi ? nt : 0;
+                    const int kend = (vi && !skip) ? nt : 0;
                     const int lm = lane % nt;
                     for (int k = 0; k < kend; ++k) {
                         int kk = k + lm;
@@ -366,12 +371,16 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         // the neighbours.  Used for weighted sums, where it beats the LDS tile.
                         const bool has = lane < nt;
                         const int jl = jt + (has ? lane : 0);
-                        float4 e = a.pl[jl];
-                        double ewj = 0.0;
-                        if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
                         uint32_t ef = a.flags[jl];
+                        const bool skip = chunk_buf && __ballot(has && (ef & 2u) == 0) == 0;
+                        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+                        double ewj = 0.0;
+                        if (!skip) {
+                            e = a.pl[jl];
+                            if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
+                        }
                         int eidx = has ? lane : -1;
-                        for (int k = 0; k < kWave; ++k) {
+                        for (int k = 0; k < (skip ? 0 : kWave); ++k) {
                             if (vi && eidx >= jmin) do {
                                 const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;
@@ -480,12 +489,15 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                                    a.out_w);
                         };
                         wave_sync();
+                        uint32_t fl = 2u;
                         if (lane < nt) {
                             tl[lane] = a.pl[jt + lane];
-                            tlf[lane] = a.flags[jt + lane];
+                            fl = a.flags[jt + lane];
+                            tlf[lane] = fl;
                         }
+                        const bool skip = chunk_buf && __ballot((fl & 2u) == 0) == 0;
                         wave_sync();
-                        const int kend = vi ? nt : 0;
+                        const int kend = (vi && !skip) ? nt : 0;
                         const int lm = lane % nt;
                         for (int k = 0; k < kend; ++k) {
                             int kk = k + lm;

@@ -369,11 +369,16 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                     if constexpr (W) {
                         // Register tile: lane L loads entry L; the entries then rotate through the
                         // wave one lane per step (DPP), so after 64 steps every lane has met every
-                        // entry.  Consecutive lanes meet different neighbours at each step, which
+                        // entry (after P steps for a tile of nt <= P entries, below).  Consecutive lanes meet different neighbours at each step, which
                         // spreads their histogram updates over many bins, and no LDS is read for
                         // the neighbours.  Used for weighted sums, where it beats the LDS tile.
-                        const bool has = lane < nt;
-                        const int jl = jt + (has ? lane : 0);
+                        // A partial tile is loaded with period P (the power of two >= nt): lane L
+                        // holds entry L mod P, a full-wave rotation keeps that pattern, and P steps
+                        // bring every entry to every lane.
+                        const int P = nt <= 1 ? 1 : 1 << (32 - __clz(nt - 1));
+                        const int le = lane & (P - 1);
+                        const bool has = le < nt;
+                        const int jl = jt + (has ? le : 0);
                         uint32_t ef = a.flags[jl];
                         const bool skip = chunk_buf && __ballot(has && (ef & 2u) == 0) == 0;
                         float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -382,8 +387,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             e = a.pl[jl];
                             if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
                         }
-                        int eidx = has ? lane : -1;
-                        for (int k = 0; k < (skip ? 0 : kWave); ++k) {
+                        int eidx = has ? le : -1;
+                        for (int k = 0; k < (skip ? 0 : P); ++k) {
                             if (vi && eidx >= jmin) do {
                                 const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;

@@ -306,7 +306,7 @@ def bench_c3(args):
     line["roofline"] = {"bound": "valu", "achieved": U * 8 / (out["stats"]["ms_count"] * 1e-3) / 1e12,
                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                         "frac": U * 8 / (out["stats"]["ms_count"] * 1e-3) / PEAK_FP32, "traffic": None,
-                        "kernel": "k_triplet3<false,false>", "kernel_ms": out["stats"]["ms_count"],
+                        "kernel": "k_triplet3<false, false, 0, false>", "kernel_ms": out["stats"]["ms_count"],
                         "flops_per_unit": 8}
     if "fp64" in res:
         line["fp64"] = {"value": res["fp64"][1]["stats"]["n_units"] / res["fp64"][0],

@@ -95,10 +95,18 @@ struct TripArgs {
     // g.h shared) and adds to its region's histogram, out_cnt/out_w + r * [ncfg][nm][4]
     int32_t nreg;
     const RegionDesc* regions;
+    // two-pass runs: with ovf set, an apex with more than kmax neighbours is appended to
+    // ovf (count in *ovf_n) and skipped instead of raising err; with items set, work item
+    // t is apex items[t] (the second pass over those apexes, with a larger kmax)
+    int32_t* ovf;
+    int32_t* ovf_n;
+    const int32_t* items;
 };
 size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);
 int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted);
+// neighbour counts of the apexes t * stride (t < nsample) of a single catalogue; -1 for buffer points
+hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int32_t* out, hipStream_t st);
 
 // Arguments of the persistent 4PCF quad kernel (quad4.hip).
 struct QuadArgs {

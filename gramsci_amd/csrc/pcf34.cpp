@@ -223,13 +223,70 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     a.g = v.g;
     a.g.m = mst;
     a.nrows = (int)rows.size();
-    a.n_items = (int)(v.n_data + v.n_rand);
-    const int occ = triplet3_occupancy(a, nb, fp64, weighted);
-    int nblocks = std::max(1, std::min(v.n_cu * occ, a.n_items));
+    const int n = (int)(v.n_data + v.n_rand);
+    a.n_items = n;
+
+    // Two passes.  The neighbour list in LDS bounds the work-groups a CU can hold, so the
+    // first pass sizes it for the 99.5th percentile of the neighbour counts of ~8192
+    // sampled apexes (+5%) and hands every apex with more neighbours to a second pass over
+    // just those apexes, with the largest list that fits.  An apex is counted in exactly
+    // one pass, so the sums are those of one pass.
+    const int kmax_full = a.kmax;
+    int kmax1 = kmax_full;
+    const int nsample = std::min(n, 8192);
+    const int stride = nsample > 0 ? n / nsample : 1;
+    if ((rc = X.i32.reserve((size_t)n + 1 + nsample))) return rc;
+    if (nsample > 0) {
+        int32_t* d_cnt = X.i32.ptr + n + 1;
+        std::vector<int32_t> kc(nsample);
+        e = launch_nbr_count(a, stride, nsample, d_cnt, v.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(kc.data(), d_cnt, nsample * sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+        if (e != hipSuccess) return hip_fail(e, "3PCF neighbour sample");
+        std::vector<int32_t> ks;
+        for (int32_t k : kc)
+            if (k >= 0) ks.push_back(k);
+        if (!ks.empty()) {
+            const size_t q = std::min(ks.size() - 1, (size_t)std::ceil(0.995 * (ks.size() - 1)));
+            std::nth_element(ks.begin(), ks.begin() + q, ks.end());
+            const int k995 = ks[q];
+            kmax1 = std::min(kmax_full, std::max(128, (int)std::ceil(1.05 * k995 / 64.0) * 64));
+        }
+    }
+    {
+        const char* ev = getenv("GS_T3_KMAX");      // tuning override (unset or empty: automatic)
+        if (ev && *ev) kmax1 = std::max(64, std::min(kmax_full, atoi(ev)));
+    }
+    const bool two_pass = kmax1 < kmax_full;
     (void)hipEventRecord(v.ev[2], v.stream);
-    if (a.n_items > 0) {
-        e = launch_triplet3(a, nb, fp64, weighted, nblocks, v.stream);
+    if (n > 0) {
+        TripArgs a1 = a;
+        if (two_pass) {
+            a1.kmax = kmax1;
+            a1.ovf = X.i32.ptr + 1;
+            a1.ovf_n = X.i32.ptr;
+            e = hipMemsetAsync(X.i32.ptr, 0, sizeof(int32_t), v.stream);
+            if (e != hipSuccess) return hip_fail(e, "3PCF overflow counter");
+        }
+        const int occ = triplet3_occupancy(a1, nb, fp64, weighted);
+        e = launch_triplet3(a1, nb, fp64, weighted, std::max(1, std::min(v.n_cu * occ, n)), v.stream);
         if (e != hipSuccess) return hip_fail(e, "3PCF kernel launch");
+        if (two_pass) {
+            int32_t nov = 0;
+            e = hipMemcpyAsync(&nov, X.i32.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+            if (e != hipSuccess) return hip_fail(e, "3PCF first pass");
+            if (nov > 0) {
+                TripArgs a2 = a;
+                a2.items = X.i32.ptr + 1;
+                a2.n_items = nov;
+                e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);   // work counters only
+                const int occ2 = triplet3_occupancy(a2, nb, fp64, weighted);
+                if (e == hipSuccess)
+                    e = launch_triplet3(a2, nb, fp64, weighted, std::max(1, std::min(v.n_cu * occ2, nov)), v.stream);
+                if (e != hipSuccess) return hip_fail(e, "3PCF second pass");
+            }
+        }
     }
     (void)hipEventRecord(v.ev[3], v.stream);
     std::vector<unsigned long long> h_out(nout);
@@ -238,7 +295,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     if (e == hipSuccess) e = hipMemcpyAsync(&err, X.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
     if (e != hipSuccess) return hip_fail(e, "3PCF kernel");
-    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", a.kmax);
+    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", kmax_full);
 
     std::vector<long double> S(nh);
     for (int k = 0; k < nh; ++k) {

@@ -158,8 +158,9 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             s_over = 0;
         }
         __syncthreads();
-        const int apex = s_item;
-        if (apex < 0) break;
+        const int item = s_item;
+        if (item < 0) break;
+        const int apex = a.items ? a.items[item] : item;
         const uint32_t fa = a.flags[apex];
         if (fa & 2u) {          // buffer point: never an apex (gramsci.F90:476)
             __syncthreads();
@@ -235,6 +236,12 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         __syncthreads();
         const int K = min(s_k, kmax);
         if (s_over) {
+            if (a.ovf) {
+                // left to the second pass, which has room for all its neighbours
+                if (tid == 0) a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
+                __syncthreads();
+                continue;
+            }
             if (tid == 0) atomicExch(a.err, 1);
         }
         const long long P = (long long)K * (K - 1) / 2;
@@ -383,6 +390,51 @@ static hipError_t launch_t3(T3Kernel kern, const TripArgs& a, size_t lds, int nb
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kWG), lds, st, a);
+    return hipGetLastError();
+}
+
+// Neighbour counts (rmin < d < rmax, exact FP64 distances) of the sampled apexes
+// t * stride, t < nsample, over the full stencil of the catalogue's grid; -1 for a
+// buffer point.  Sizes the first pass's neighbour list (one thread per apex).
+__global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int nsample, int32_t* __restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsample) return;
+    const int apex = t * stride;
+    if (a.flags[apex] & 2u) {
+        out[t] = -1;
+        return;
+    }
+    const BinDesc& B = *a.bins;
+    const double t_lo = B.t64[0], t_hi = B.t64[B.nb + 1];
+    const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
+    const double inv_h = 1.0 / a.g.h;
+    const double4 Pa = a.pw[apex];
+    const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_h), 0), nx - 1);
+    const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), ny - 1);
+    const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nz - 1);
+    int cnt = 0;
+    for (int r = 0; r < a.nrows; ++r) {
+        const Row rw = a.rows[r];
+        const int by = cy + rw.dy, bz = cz + rw.dz;
+        if (by < 0 || by >= ny || bz < 0 || bz >= nz) continue;
+        const int bx0 = max(cx + rw.x0, 0), bx1 = min(cx + rw.x1, nx - 1);
+        if (bx0 > bx1) continue;
+        const int rb = (bz * ny + by) * nx;
+        const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
+        for (int j = js; j < je; ++j) {
+            const double4 Pj = a.pw[j];
+            const double s = exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z);
+            cnt += (s > t_lo && s < t_hi) ? 1 : 0;
+        }
+    }
+    out[t] = cnt;
+}
+
+hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int32_t* out, hipStream_t st)
+{
+    if (nsample <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nbr_count, dim3((nsample + 255) / 256), dim3(256), 0, st, a, stride, nsample, out);
     return hipGetLastError();
 }
 

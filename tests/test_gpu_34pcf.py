@@ -132,6 +132,26 @@ def test_3pcf_clustered_log_weighted_buffer(eng):
     check3(eng, xyz, None, 6000, O.Params(3.0, 12.0, 4, 1, False))
 
 
+@pytest.mark.parametrize("nmu", [1, 4])
+def test_3pcf_two_pass_overflow(eng, monkeypatch, nmu):
+    # a first-pass neighbour list of 128 entries (about the mean neighbour count here): the
+    # apexes with more neighbours go to the second pass, and every apex is counted once
+    rng = np.random.default_rng(11)
+    d = C.clustered(6000, 120.0, 75)
+    r = C.uniform(12000, 120.0, 76)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(6000, 0.2, 77), np.ones(12000)])
+    buf = (rng.random(len(xyz)) < 0.1).astype(np.int32)
+    p = O.Params(2.0, 14.0, 5, nmu, False)
+    eng.set_catalogue(xyz, w, 6000, buf)
+    base = eng.pcf3(p.rmin, p.rmax, p.nbins, nmu=nmu, precision="fp32")
+    monkeypatch.setenv("GS_T3_KMAX", "128")
+    ref, outs = check3(eng, xyz, w, 6000, p, buf=buf)
+    for o in outs:
+        assert np.array_equal(o["cnt"], base["cnt"]) and np.array_equal(o["wsum"], base["wsum"])
+        assert o["stats"]["n_units"] == base["stats"]["n_units"]
+
+
 def test_3pcf_config3_density_slab(eng):
     # a 150^3 slab at the density of the 3PCF configuration (600k points in 500^3)
     d = C.clustered(5400, 150.0, 81)

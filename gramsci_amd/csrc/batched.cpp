@@ -38,7 +38,7 @@ void batch_release(GsBatch& b)
     b.d_off.release(); b.d_row_base.release(); b.cell_of.release(); b.count.release(); b.cell_start.release();
     b.nch.release(); b.row_chunk.release(); b.scan_tmp.release(); b.queue.release(); b.orig.release();
     b.d_regions.release(); b.params.release(); b.pw.release(); b.pl.release(); b.flags.release(); b.out.release();
-    b.xyz.release(); b.w.release(); b.buf.release();
+    b.xyz.release(); b.w.release(); b.buf.release(); b.ovf.release();
     b.valid = false;
 }
 
@@ -124,7 +124,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     const bool grow = (size_t)n > B.cell_of.cap || (size_t)(cells + 1) > B.count.cap ||
                       (size_t)(cells + 1) > B.cell_start.cap || (size_t)(rows_tot + 1) > B.nch.cap ||
                       (size_t)(rows_tot + 1) > B.row_chunk.cap || (size_t)scan_scratch_size(scan_n) > B.scan_tmp.cap ||
-                      nout > B.out.cap;
+                      nout > B.out.cap || (kind == 3 && (size_t)(n + 1) > B.ovf.cap) || B.queue.cap < 32;
     if (grow) {
         for (auto& pl : B.plans) {
             if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
@@ -135,7 +135,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     }
     if ((rc = B.cell_of.reserve(n)) || (rc = B.count.reserve(cells + 1)) || (rc = B.cell_start.reserve(cells + 1)) ||
         (rc = B.nch.reserve(rows_tot + 1)) || (rc = B.row_chunk.reserve(rows_tot + 1)) ||
-        (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(16)) ||
+        (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(32)) ||
+        (kind == 3 && (rc = B.ovf.reserve(n + 1))) ||
         (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n)) || (rc = B.flags.reserve(n)) ||
         (rc = B.out.reserve(nout)))
         return rc;
@@ -216,7 +217,33 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         ta.nreg = nreg;
         ta.regions = d_reg;
     }
-    const int occ = kind == 3 ? triplet3_occupancy(ta, nb, fp64, weighted)
+    // 3PCF in two captured passes: the first with a neighbour list sized for twice the largest
+    // mean neighbour count of any region's box (apexes with more neighbours are queued), the
+    // second over the queued apexes with the largest list that fits (count read on the device)
+    TripArgs ta1 = ta, ta2 = ta;
+    int nblocks2 = 0;
+    if (kind == 3) {
+        const double shell = 4.0 / 3.0 * M_PI * (std::pow(p.rmax, 3) - std::pow(std::max(p.rmin, 0.0), 3));
+        double kmean = 0.0;
+        for (int r = 0; r < nreg; ++r) {
+            double vol = 1.0;
+            for (int k = 0; k < 3; ++k) vol *= std::max(B.bbox[6 * r + 3 + k] - B.bbox[6 * r + k], p.rmax);
+            kmean = std::max(kmean, (double)(B.off[r + 1] - B.off[r]) * shell / vol);
+        }
+        int kmax1 = std::min(ta.kmax, std::max(128, (int)std::ceil(2.0 * kmean / 64.0) * 64));
+        const char* ev = getenv("GS_T3_KMAX");      // tuning override (unset or empty: automatic)
+        if (ev && *ev) kmax1 = std::max(64, std::min(ta.kmax, atoi(ev)));
+        if (kmax1 < ta.kmax) {
+            ta1.kmax = kmax1;
+            ta1.ovf_n = B.ovf.ptr;
+            ta1.ovf = B.ovf.ptr + 1;
+            ta2.items = B.ovf.ptr + 1;
+            ta2.n_items_dev = B.ovf.ptr;
+            ta2.queue = B.queue.ptr + 16;
+            nblocks2 = v.n_cu * triplet3_occupancy(ta2, nb, fp64, weighted);
+        }
+    }
+    const int occ = kind == 3 ? triplet3_occupancy(ta1, nb, fp64, weighted)
                               : std::max(1, pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true));
     const int nblocks = kind == 3 ? std::max(1, (int)std::min<int64_t>((int64_t)v.n_cu * occ, std::max<int64_t>(n, 1)))
                                   : v.n_cu * occ;
@@ -224,7 +251,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     // capture: clear, count sort into cells, scans, scatter, row chunks, pair or triplet kernel
     e = hipStreamBeginCapture(v.stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
-    if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 16, 0, v.stream);
+    if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 32, 0, v.stream);
+    if (e == hipSuccess && nblocks2 > 0) e = launch_fill_u32(B.ovf.ptr, 1, 0, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.out.ptr, (int64_t)nout * 2, 0, v.stream);
     if (e == hipSuccess)
         e = launch_cell_count_b(B.xyz.ptr, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
@@ -240,7 +268,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         if (e == hipSuccess) e = exclusive_scan(B.nch.ptr, B.row_chunk.ptr, rows_tot + 1, B.scan_tmp.ptr, v.stream);
         if (e == hipSuccess) e = launch_pair2(a, nb, nm, fp64, rsd, weighted, p.log_bins != 0, nblocks, kWG, v.stream);
     } else if (e == hipSuccess && n > 0) {
-        e = launch_triplet3(ta, nb, fp64, weighted, nblocks, v.stream);
+        e = launch_triplet3(ta1, nb, fp64, weighted, nblocks, v.stream);
+        if (e == hipSuccess && nblocks2 > 0) e = launch_triplet3(ta2, nb, fp64, weighted, nblocks2, v.stream);
     }
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(v.stream, &graph);

@@ -43,6 +43,7 @@ struct GsBatch {
     std::vector<long double> wd, wr;     // per-region weight sums
     double wmax = 1.0, xmax = 0.0;
     DevBuf<int32_t> d_off, d_row_base, cell_of, count, cell_start, nch, row_chunk, scan_tmp, queue, orig;
+    DevBuf<int32_t> ovf;                 // 3PCF: overflow count + apexes left to the second pass
     DevBuf<unsigned char> d_regions, params;
     DevBuf<double4> pw;
     DevBuf<float4> pl;

@@ -150,9 +150,10 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const int xcd = blockIdx.x & 7;
     int tried = 0;
 
+    const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
     while (true) {
         if (tid == 0) {
-            const int it = next_item(a.queue, a.n_items, xcd, tried);
+            const int it = next_item(a.queue, n_items, xcd, tried);
             s_item = it;
             s_k = 0;
             s_over = 0;

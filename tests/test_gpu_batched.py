@@ -106,6 +106,24 @@ def test_batched_3pcf_equals_separate_launches_and_replays(eng, batch):
         assert np.array_equal(a["wsum"][r][:, 0], sep[r]["wsum"]), r
 
 
+def test_batched_3pcf_two_pass_overflow(eng, batch, monkeypatch):
+    # a 96-entry first-pass neighbour list: most apexes go to the captured second pass,
+    # whose apex count is read on the device; results equal the oracle's per region
+    xyz, buf, off, ndata = batch
+    w = C.lognormal_weights(len(xyz), 0.2, 29)
+    p = O.Params(2.0, 8.0, 4, 1, False)
+    monkeypatch.setenv("GS_T3_KMAX", "96")
+    eng.set_catalogue_batched(xyz, w, buf, off, ndata)        # fresh plans read the override
+    out = eng.pcf3_batched(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32", cell_div=2)
+    again = eng.pcf3_batched(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32", cell_div=2)   # replay
+    assert np.array_equal(out["cnt"], again["cnt"]) and np.array_equal(out["wsum"], again["wsum"])
+    for r in range(len(ndata)):
+        sl = slice(off[r], off[r + 1])
+        ref = O.pcf3(xyz[sl], w[sl], int(ndata[r]), p, buf=buf[sl])
+        assert np.array_equal(out["cnt"][r].reshape(ref["cnt"].shape), ref["cnt"]), r
+        assert _close(out["n2"][r].reshape(-1), ref["n2"].reshape(-1))
+
+
 def test_batch_survives_single_catalogue_uploads(eng, batch):
     # the batch keeps its own copy of the catalogue: uploading a single catalogue on the
     # same context (smaller or larger) changes neither replays of captured pipelines nor

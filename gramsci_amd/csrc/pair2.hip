@@ -209,6 +209,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     const double ms = B.mu_scale;
     const int nbm = nb * nm;
     const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
+    const float nbm1f = (float)(nb - 1), nmm1f = (float)(nm - 1);
     const int shift = B.fx_shift2;
     const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
     const int n_chunks = batched ? *a.n_chunks_dev : a.n_chunks;
@@ -394,8 +395,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;
                                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                                 if (!(s > rej_lo && s < rej_hi)) break;
-                                int b = (int)bin_estimate(s, LOGB, ua, ub);
-                                b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                                int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
                                 const float2 ok = okb[b];
                                 bool safe = (s >= ok.x) && (s < ok.y);
                                 int m = 0, mr = 0;
@@ -409,12 +409,11 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
                                     const float dt = ms32 * fmaf(mu_ew, r, fmaf(mu_em, rm, mu_c));
                                     // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
-                                    // nearest integer is no edge or is farther than the error bound dt < 1/2
-                                    const float rt = rintf(t);
-                                    const bool edge = (rt >= 1.0f) && (rt <= (float)(nm - 1));
-                                    safe = safe && (dt < 0.5f) && (!edge || fabsf(t - rt) >= dt);
-                                    const int mi = (int)floorf(t);
-                                    m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
+                                    // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
+                                    // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
+                                    const float edge = __builtin_amdgcn_fmed3f(rintf(t), 1.0f, nmm1f);
+                                    safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
+                                    m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                                     mr = nm - 1 - m;
                                 }
                                 if (!safe) {
@@ -458,8 +457,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             const float dy = e.y - yi_r, dz = e.z - zi_r;
                             const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                             if (!(s > rej_lo && s < rej_hi)) return;
-                            int b = (int)bin_estimate(s, LOGB, ua, ub);
-                            b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                            int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
                             const float2 ok = okb[b];
                             bool safe = (s >= ok.x) && (s < ok.y);
                             int m = 0, mr = 0;
@@ -473,12 +471,11 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
                                 const float dt = ms32 * fmaf(mu_ew, r, fmaf(mu_em, rm, mu_c));
                                 // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
-                                // nearest integer is no edge or is farther than the error bound dt < 1/2
-                                const float rt = rintf(t);
-                                const bool edge = (rt >= 1.0f) && (rt <= (float)(nm - 1));
-                                safe = safe && (dt < 0.5f) && (!edge || fabsf(t - rt) >= dt);
-                                const int mi = (int)floorf(t);
-                                m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
+                                // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
+                                // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
+                                const float edge = __builtin_amdgcn_fmed3f(rintf(t), 1.0f, nmm1f);
+                                safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
+                                m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                                 mr = nm - 1 - m;
                             }
                             if (!safe) {

@@ -210,6 +210,13 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     const int nbm = nb * nm;
     const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
     const float nbm1f = (float)(nb - 1), nmm1f = (float)(nm - 1);
+    // mu error bound dt = ms * (mu_ew / |sep| + mu_em / |mid| + mu_c), with ms folded into the
+    // constants (their 4x margin dwarfs the reassociation).  In the weighted kernel, whose
+    // residency LDS limits, dt_c and ub are kept in VGPRs so that each FMA of the loop reads at
+    // most one scalar operand; elsewhere the extra VGPRs would cost residency.
+    const float dt_ew = ms32 * mu_ew, dt_em = ms32 * mu_em;
+    float dt_c = ms32 * mu_c, ubv = ub;
+    if constexpr (W) asm volatile("" : "+v"(dt_c), "+v"(ubv));
     const int shift = B.fx_shift2;
     const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
     const int n_chunks = batched ? *a.n_chunks_dev : a.n_chunks;
@@ -395,7 +402,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;
                                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                                 if (!(s > rej_lo && s < rej_hi)) break;
-                                int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
+                                int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
                                 const float2 ok = okb[b];
                                 bool safe = (s >= ok.x) && (s < ok.y);
                                 int m = 0, mr = 0;
@@ -407,7 +414,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
                                     const float rm = __builtin_amdgcn_rsqf(M2);
                                     const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
-                                    const float dt = ms32 * fmaf(mu_ew, r, fmaf(mu_em, rm, mu_c));
+                                    const float dt = fmaf(dt_ew, r, fmaf(dt_em, rm, dt_c));
                                     // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
                                     // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
                                     // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
@@ -457,7 +464,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             const float dy = e.y - yi_r, dz = e.z - zi_r;
                             const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                             if (!(s > rej_lo && s < rej_hi)) return;
-                            int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
+                            int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
                             const float2 ok = okb[b];
                             bool safe = (s >= ok.x) && (s < ok.y);
                             int m = 0, mr = 0;
@@ -469,7 +476,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
                                 const float rm = __builtin_amdgcn_rsqf(M2);
                                 const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
-                                const float dt = ms32 * fmaf(mu_ew, r, fmaf(mu_em, rm, mu_c));
+                                const float dt = fmaf(dt_ew, r, fmaf(dt_em, rm, dt_c));
                                 // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
                                 // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
                                 // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)

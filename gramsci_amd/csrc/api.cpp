@@ -234,9 +234,10 @@ int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc
     B.mu_ew = (float)(2.0 * std::sqrt(3.0) * ew * 4.0);
     B.mu_em = (float)(2.0 * std::sqrt(3.0) * em * 4.0);
     B.mu_c = (float)std::ldexp(1.0, -18);
-    // fixed point: the largest pair / triple product stays below 2^61
+    // fixed point: the largest pair product stays below 2^58 (the pair kernel sums runs of up
+    // to 33 of them in one 64-bit register), the largest triple product below 2^61
     const double wm = std::max(wmax, 1e-300);
-    B.fx_shift2 = 61 - (int)std::ceil(std::log2(wm * wm * (1.0 + 1e-12)));
+    B.fx_shift2 = 58 - (int)std::ceil(std::log2(wm * wm * (1.0 + 1e-12)));
     B.fx_shift3 = 61 - (int)std::ceil(std::log2(wm * wm * wm * (1.0 + 1e-12)));
     if (B.fx_shift2 > 1000) B.fx_shift2 = 1000;
     if (B.fx_shift3 > 1000) B.fx_shift3 = 1000;

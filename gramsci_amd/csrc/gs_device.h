@@ -96,6 +96,14 @@ __device__ __forceinline__ void quantise(double x, uint32_t& lo, uint32_t& hi)
     lo = (uint32_t)fma(-h, 0x1p32, x);
 }
 
+// floor(x) as one 64-bit integer (the register pair lo, hi of quantise)
+__device__ __forceinline__ unsigned long long quantise64(double x)
+{
+    uint32_t lo, hi;
+    quantise(x, lo, hi);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // largest r with off[r] <= v over off[0..n-1] (off non-decreasing)
 __device__ __forceinline__ int upper_index(const int32_t* off, int n, int v)
 {

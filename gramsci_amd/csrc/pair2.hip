@@ -228,32 +228,29 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     // neighbours (adjacent points of one cell, data before randoms) often fall in the same
     // bin; their updates are merged in registers and reach the LDS histogram with one set of
     // atomics when the bin changes, which relieves same-address LDS atomic conflicts.
+    // The run's sum is one 64-bit integer: products are below 2^58 (fx_shift2) and a run ends
+    // after at most 33 increments, so it cannot overflow; the flush splits it at bit 32 into
+    // the two LDS halves, which keeps value = hi * 2^32 + lo.  The update is written with
+    // selects, not branches, so that no register copies are needed where paths merge.
     int ck = -1;
     uint32_t cc = 0;
-    unsigned long long clo = 0, chi = 0;
+    unsigned long long cq = 0;
     auto flush = [&]() {
         if (ck >= 0) {
             atomicAdd(&hc[(ck << rsh) + rep], cc);
             if constexpr (W) {
-                atomicAdd(&hw[2 * ck], clo);
-                atomicAdd(&hw[2 * ck + 1], chi);
+                atomicAdd(&hw[2 * ck], cq & 0xffffffffull);
+                atomicAdd(&hw[2 * ck + 1], cq >> 32);
             }
             ck = -1;
         }
     };
-    auto put = [&](int k, uint32_t c, uint32_t qlo, uint32_t qhi) {
-        if (k != ck) {
-            flush();
-            ck = k;
-            cc = 0;
-            clo = 0;
-            chi = 0;
-        }
-        cc += c;
-        if constexpr (W) {
-            clo += (unsigned long long)qlo * c;
-            chi += (unsigned long long)qhi * c;
-        }
+    auto put = [&](int k, uint32_t c, unsigned long long q) {   // c in {1, 2}
+        const bool nw = (k != ck) || (cc >= 32u);
+        if (nw) flush();
+        cc = (nw ? 0u : cc) + c;
+        if constexpr (W) cq = (nw ? 0ull : cq) + (q << (c - 1));
+        ck = k;
     };
     unsigned long long bound = 0;   // LDS increments issued by this wave since its last drain
     // LDS bin increments between drains, over all waves sharing the histogram, stay below 2^31,
@@ -436,12 +433,12 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 {
                                     const int cj = (ef & 2) ? 0 : 1;
                                     const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
-                                    uint32_t qlo = 0, qhi = 0;
-                                    if constexpr (W) quantise(wi * ewj, qlo, qhi);
+                                    unsigned long long q = 0;
+                                    if constexpr (W) q = quantise64(wi * ewj);
                                     if constexpr (W && !RSD) {
-                                        if (ci + cj) put(base, (uint32_t)(ci + cj), qlo, qhi);
+                                        if (ci + cj) put(base, (uint32_t)(ci + cj), q);
                                     } else if (W && mirror && mr == nm - 1 - m) {
-                                        put(base + m, 1u, qlo, qhi);   // stands for i->j at m and j->i at nm-1-m
+                                        put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
                                     } else {
                                         accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, wi * ewj,
                                                            a.out_cnt, a.out_w);

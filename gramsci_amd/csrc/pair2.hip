@@ -392,9 +392,12 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             e = a.pl[jl];
                             if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
                         }
-                        int eidx = has ? le : -1;
+                        // entry index and flags in one rotated register: ep = eidx << 2 | flags (-1: no
+                        // entry), so the self-row test eidx >= jmin is ep >= jmin << 2
+                        int ep = has ? (le << 2) | (int)(ef & 3u) : -1;
+                        const int jmin4 = jmin << 2;
                         for (int k = 0; k < (skip ? 0 : P); ++k) {
-                            if (vi && eidx >= jmin) do {
+                            if (vi && ep >= jmin4) do {
                                 const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;
                                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -420,24 +423,28 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                                     mr = nm - 1 - m;
                                 }
+                                bool mirrored = true;     // the FP32 mu bins are mirror images by construction
                                 if (!safe) {
                                     ++st.rechk;
-                                    const double4 Pj = a.pw[jt + eidx];
+                                    const double4 Pj = a.pw[jt + (ep >> 2)];
                                     Pi = a.pw[i];
                                     const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
                                     if (!(se > t_lo && se < t_hi)) break;
                                     b = exact_bin(se, t64, nb);
-                                    if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); break; }
-                                    if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                    if (b == nb) { st.over += (uint64_t)(ci + ((ep & 2) ? 0 : 1)); break; }
+                                    if constexpr (RSD) {
+                                        exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                        mirrored = mr == nm - 1 - m;
+                                    }
                                 }
                                 {
-                                    const int cj = (ef & 2) ? 0 : 1;
-                                    const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
+                                    const int cj = (ep & 2) ? 0 : 1;
+                                    const int base = ((ep & 1) ? cb1 : cb0) + __umul24(b, nm);
                                     unsigned long long q = 0;
                                     if constexpr (W) q = quantise64(wi * ewj);
                                     if constexpr (W && !RSD) {
                                         if (ci + cj) put(base, (uint32_t)(ci + cj), q);
-                                    } else if (W && mirror && mr == nm - 1 - m) {
+                                    } else if (W && mirror && mirrored) {
                                         put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
                                     } else {
                                         accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, wi * ewj,
@@ -448,8 +455,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             // rotate the register tile by one lane (all lanes active here)
                             e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
                             if constexpr (W) ewj = rot_lane(ewj);
-                            ef = rot_lane(ef);
-                            eidx = rot_lane(eidx);
+                            ep = rot_lane(ep);
                         }
                     } else {
                         // Unweighted: LDS tile with the rotated schedule of the FP64 path (a partial

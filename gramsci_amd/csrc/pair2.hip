@@ -68,12 +68,13 @@ __device__ __forceinline__ double rot_lane(double v)
 }
 
 // One pair's histogram update.  base = class offset + b*nm (flat bin index
-// without the mu bin); counts live at ((base + m) << rsh) + rep.  Returns the
-// number of LDS bin increments made.
+// without the mu bin); counts live at ((base + m) << rsh) + rep.  mirrored:
+// mr == nm-1-m (always so for the FP32 mu bins; tested after an FP64 recheck).
+// Returns the number of LDS bin increments made.
 template <bool RSD, bool W>
 __device__ __forceinline__ int accumulate(uint32_t* hc, unsigned long long* hw, int rsh, int rep, int nm,
-                                          bool mirror, int base, int m, int mr, int ci, int cj, double x,
-                                          unsigned long long* out_cnt, unsigned long long* out_w)
+                                          bool mirror, int base, int m, int mr, bool mirrored, int ci, int cj,
+                                          double x, unsigned long long* out_cnt, unsigned long long* out_w)
 {
     uint32_t qlo = 0, qhi = 0;
     if constexpr (W) quantise(x, qlo, qhi);
@@ -88,7 +89,7 @@ __device__ __forceinline__ int accumulate(uint32_t* hc, unsigned long long* hw, 
         const int c = ci + cj;
         if (c) add(base, (uint32_t)c);
         return c;
-    } else if (mirror && mr == nm - 1 - m) {
+    } else if (mirror && mirrored) {
         add(base + m, 1u);    // mirror-implied: stands for i->j at m and j->i at nm-1-m
         return 1;
     } else if (mirror) {
@@ -337,7 +338,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                     const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
                     double x = 0.0;
                     if constexpr (W) x = wi * wj;
-                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, x, a.out_cnt, a.out_w);
+                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mr == nm - 1 - m, ci, cj, x,
+                                       a.out_cnt, a.out_w);
                 };
                 if constexpr (FP64) {
                     wave_sync();
@@ -447,8 +449,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     } else if (W && mirror && mirrored) {
                                         put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
                                     } else {
-                                        accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, wi * ewj,
-                                                           a.out_cnt, a.out_w);
+                                        accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mirrored, ci, cj,
+                                                           wi * ewj, a.out_cnt, a.out_w);
                                     }
                                 }
                             } while (false);
@@ -471,6 +473,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             const float2 ok = okb[b];
                             bool safe = (s >= ok.x) && (s < ok.y);
                             int m = 0, mr = 0;
+                            bool mirrored = true;
                             if constexpr (RSD) {
                                 const float r = __builtin_amdgcn_rsqf(s);
                                 // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
@@ -496,12 +499,15 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 if (!(se > t_lo && se < t_hi)) return;
                                 b = exact_bin(se, t64, nb);
                                 if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); return; }
-                                if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                if constexpr (RSD) {
+                                    exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                    mirrored = mr == nm - 1 - m;
+                                }
                             }
                             const int cj = (ef & 2) ? 0 : 1;
                             const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
-                            accumulate<RSD, false>(hc, hw, rsh, rep, nm, mirror, base, m, mr, ci, cj, 0.0, a.out_cnt,
-                                                   a.out_w);
+                            accumulate<RSD, false>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mirrored, ci, cj, 0.0,
+                                                   a.out_cnt, a.out_w);
                         };
                         wave_sync();
                         uint32_t fl = 2u;

@@ -25,7 +25,7 @@ using namespace dev;
 size_t adjoint2_lds_bytes(int nb, int nm, bool fp64)
 {
     size_t b = 0;
-    b += (size_t)nb * nm * 8 * 2;                              // g2, g3
+    b += (size_t)nb * nm * 8 * 2;                              // g2, g2 + g3
     b += ((size_t)(kMaxBins + 1) * 8);                          // ok_lo/ok_hi
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;           // t64
     const int waves = kWG / kWave;
@@ -45,8 +45,8 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
     const int wv = tid / kWave;
 
     unsigned char* p = smem;
-    double* g2 = (double*)p;
-    double* g3 = g2 + nb * nm;
+    double* g2 = (double*)p;                // dL/dN2 per (bin, mu)
+    double* g23 = g2 + nb * nm;             // dL/dN2 + dL/dN3 (a random-random pair feeds both)
     p += (size_t)nb * nm * 16;
     float2* okb = (float2*)p;
     p += (size_t)(kMaxBins + 1) * 8;
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
     double* tw = FP64 ? (tz + kWave) : (double*)(tl + kWave);
     uint32_t* tf = (uint32_t*)(tw + kWave);
 
-    for (int k = tid; k < nb * nm; k += kWG) { g2[k] = a.g2[k]; g3[k] = a.g3[k]; }
+    for (int k = tid; k < nb * nm; k += kWG) { g2[k] = a.g2[k]; g23[k] = a.g2[k] + a.g3[k]; }
     for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
     for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
     __syncthreads();
@@ -70,7 +70,9 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
     const bool logb = B.log_bins != 0;
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const double ms = B.mu_scale;
-    const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
+    const float ms32 = B.ms32;
+    const float dt_ew = ms32 * B.mu_ew, dt_em = ms32 * B.mu_em, dt_c = ms32 * B.mu_c;   // mu error bound
+    const float nbm1f = (float)(nb - 1), nmm1f = (float)(nm - 1);
     const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
     const int nrow = ny * nz;
     const float h32 = a.g.h32;
@@ -149,8 +151,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
                         const float dy = tj.y - yi_r, dz = tj.z - zi_r;
                         const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         if (!(s > rej_lo && s < rej_hi)) continue;
-                        b = (int)bin_estimate(s, logb, ua, ub);
-                        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                        b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, logb, ua, ub), 0.0f, nbm1f);
                         const float2 ok = okb[b];
                         bool safe = (s >= ok.x) && (s < ok.y);
                         if constexpr (RSD) {
@@ -158,15 +159,12 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
                             const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
                             const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
                             const float rm = __builtin_amdgcn_rsqf(M2), rsq = __builtin_amdgcn_rsqf(s);
-                            const float t = fmaf(D * rm * rsq, ms32, ms32);
-                            const float tfl = floorf(t);
-                            const float fr = t - tfl;
-                            const float dt = ms32 * fmaf(mu_ew, rsq, fmaf(mu_em, rm, mu_c));
-                            const int mi = (int)tfl;
-                            const bool near_lo = (fr < dt) && (mi >= 1) && (mi <= nm - 1);
-                            const bool near_hi = (fr > 1.0f - dt) && (mi >= 0) && (mi <= nm - 2);
-                            safe = safe && !near_lo && !near_hi && (dt < 0.25f);
-                            m = mi < 0 ? 0 : (mi > nm - 1 ? nm - 1 : mi);
+                            const float t = fmaf(D * rm * rsq, ms32, ms32);       // (theta + 1) * mu_scale
+                            const float dt = fmaf(dt_ew, rsq, fmaf(dt_em, rm, dt_c));
+                            // certain when the nearest mu edge (an integer 1..nm-1 of t) is farther than dt
+                            const float edge = __builtin_amdgcn_fmed3f(rintf(t), 1.0f, nmm1f);
+                            safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
+                            m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                             mr = nm - 1 - m;
                         }
                         if (!safe) {
@@ -182,9 +180,10 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
                     const int cj = (fj & 2) ? 0 : 1;
                     const bool rr = ri && (fj & 1);
                     const int o = b * nm;
+                    const double* gt = rr ? g23 : g2;
                     double c = 0.0;
-                    if (ci) c += rr ? g2[o + m] + g3[o + m] : g2[o + m];
-                    if (cj) c += rr ? g2[o + mr] + g3[o + mr] : g2[o + mr];
+                    if (ci) c += gt[o + m];
+                    if (cj) c += gt[o + mr];
                     G = fma(c, tw[k], G);
                 }
             }

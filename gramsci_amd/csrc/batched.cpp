@@ -114,6 +114,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         if (nm > 1) ntab = normal_table(nm);
         ta.ncfg = nb * (nb + 1) * (nb + 2) / 6;
         ta.nm = nm;
+        ta.log_bins = p.log_bins != 0;
         if ((rc = plan_triplet_lds(ta, nb, fp64, weighted))) return rc;
         nh = ta.ncfg * nm * 4;
     }

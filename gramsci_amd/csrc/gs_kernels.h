@@ -102,6 +102,7 @@ struct TripArgs {
     int32_t* ovf_n;
     const int32_t* items;
     const int32_t* n_items_dev;     // if set, the number of work items is read here (captured second pass)
+    int32_t log_bins;               // selects the kernel variant (log or linear bin estimate)
 };
 size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);

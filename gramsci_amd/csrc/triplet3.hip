@@ -59,7 +59,7 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
 // LDS histogram, the common case, runs with GEN 0).
 enum { kT3Rsd = 1, kT3Equi = 2 };
 
-template <bool FP64, bool W, int MODE, bool GEN>
+template <bool FP64, bool W, int MODE, bool GEN, bool LOGB>
 __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -74,6 +74,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int rep = lane & (reps - 1);
+    const int rsh = 31 - __clz(reps);       // reps is a power of two
 
     unsigned char* p = smem;
     uint32_t* hc = (uint32_t*)p;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const double ms = B.mu_scale;
 
     const float ua = B.ua, ub = B.ub;
-    const bool logb = B.log_bins != 0;
+    const float nbm1f = (float)(nb - 1);
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
     const int shift = B.fx_shift3;
@@ -209,8 +210,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const double s = exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z);
                 if (!(s > t_lo && s < t_hi)) continue;
                 const float sf = (float)s;
-                int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub));
-                b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(sf, LOGB, ua, ub), 0.0f, nbm1f);
                 while (b > 0 && s < t64[b]) --b;
                 while (b < nb && s >= t64[b + 1]) ++b;
                 if (b == nb) { ++over; continue; }
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             }
             if (tid == 0) atomicExch(a.err, 1);
         }
-        const long long P = (long long)K * (K - 1) / 2;
+        const int P = K * (K - 1) / 2;          // K <= 4096, so P < 2^31
         if (tid == 0) units += (unsigned long long)P;
         const int ra = (int)(fa & 1u);
         const double wa = Pa.w;
@@ -253,9 +253,9 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         // ---- 2. neighbour pairs
         // flat pair index pi -> (u, v), u < v, row u holding v = u+1 .. K-1
         int u = 0;
-        long long rem = tid;
+        int rem = tid;
         while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
-        for (long long pi = tid; pi < P; pi += kWG) {
+        for (int pi = tid; pi < P; pi += kWG) {
             const int v = u + 1 + (int)rem;
             uint32_t pu, pv;
             int b3 = -1;
@@ -267,8 +267,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
                     if (s > t_lo && s < t_hi) {
                         const float sf = (float)s;
-                        int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(sf), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(sf), ua, ub));
-                        b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                        int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(sf, LOGB, ua, ub), 0.0f, nbm1f);
                         while (b > 0 && s < t64[b]) --b;
                         while (b < nb && s >= t64[b + 1]) ++b;
                         if (b == nb) ++over; else b3 = b;
@@ -281,10 +280,9 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const float dx = Rv.x - Ru.x, dy = Rv.y - Ru.y, dz = Rv.z - Ru.z;
                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 if ((!equi || ((pu ^ pv) & 255u) == 0) && s > rej_lo && s < rej_hi) {
-                    int b = (int)(logb ? fmaf(__builtin_amdgcn_logf(s), ua, ub) : fmaf(__builtin_amdgcn_sqrtf(s), ua, ub));
-                    b = b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+                    const int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
                     const float2 ok = okb[b];
-                    if (s >= ok.x && s < ok.y) {
+                    if ((s >= ok.x) & (s < ok.y)) {
                         b3 = b;
                     } else {
                         ++rechk;
@@ -306,7 +304,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 return (wa * ldexp(wl[xf ? x : y], -shift)) * wl[xf ? y : x];
             };
             if (b3 >= 0) {
-                const int cfg = equi ? b3 : (int)tab[((pu & 255u) * nb + (pv & 255u)) * nb + b3];
+                const int cfg = equi ? b3 : (int)tab[__umul24(__umul24(pu & 255u, nb) + (pv & 255u), nb) + b3];
                 int mu = 0;
                 if (nm > 1) {
                     const int mu_u = (int)((pu >> 9) & 127u), mu_v = (int)((pv >> 9) & 127u);
@@ -316,7 +314,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const int hk = (cfg * nm + mu) * 4 + cls;
                 ++binned;
                 if (lh) {
-                    atomicAdd(&hc[hk * reps + rep], 1u);
+                    atomicAdd(&hc[(hk << rsh) + rep], 1u);
                     if constexpr (W) {
                         uint32_t qlo, qhi;
                         quantise(triple(u, v), qlo, qhi);
@@ -364,24 +362,31 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 
 using T3Kernel = void (*)(TripArgs);
 
-template <bool FP64, bool W, int MODE>
+template <bool FP64, bool W, int MODE, bool LOGB>
 static T3Kernel pick_t3m(bool gen)
 {
-    return gen ? k_triplet3<FP64, W, MODE, true> : k_triplet3<FP64, W, MODE, false>;
+    return gen ? k_triplet3<FP64, W, MODE, true, LOGB> : k_triplet3<FP64, W, MODE, false, LOGB>;
 }
 
-template <bool FP64, bool W>
+template <bool FP64, bool W, bool LOGB>
 static T3Kernel pick_t3w(const TripArgs& a)
 {
     const bool gen = a.nreg > 0 || !a.lds_hist;
-    if (a.equi) return a.nm > 1 ? pick_t3m<FP64, W, kT3Equi | kT3Rsd>(gen) : pick_t3m<FP64, W, kT3Equi>(gen);
-    return a.nm > 1 ? pick_t3m<FP64, W, kT3Rsd>(gen) : pick_t3m<FP64, W, 0>(gen);
+    if (a.equi)
+        return a.nm > 1 ? pick_t3m<FP64, W, kT3Equi | kT3Rsd, LOGB>(gen) : pick_t3m<FP64, W, kT3Equi, LOGB>(gen);
+    return a.nm > 1 ? pick_t3m<FP64, W, kT3Rsd, LOGB>(gen) : pick_t3m<FP64, W, 0, LOGB>(gen);
+}
+
+template <bool LOGB>
+static T3Kernel pick_t3l(const TripArgs& a, bool fp64, bool weighted)
+{
+    if (fp64) return weighted ? pick_t3w<true, true, LOGB>(a) : pick_t3w<true, false, LOGB>(a);
+    return weighted ? pick_t3w<false, true, LOGB>(a) : pick_t3w<false, false, LOGB>(a);
 }
 
 static T3Kernel pick_t3(const TripArgs& a, bool fp64, bool weighted)
 {
-    if (fp64) return weighted ? pick_t3w<true, true>(a) : pick_t3w<true, false>(a);
-    return weighted ? pick_t3w<false, true>(a) : pick_t3w<false, false>(a);
+    return a.log_bins ? pick_t3l<true>(a, fp64, weighted) : pick_t3l<false>(a, fp64, weighted);
 }
 
 static hipError_t launch_t3(T3Kernel kern, const TripArgs& a, size_t lds, int nblocks, hipStream_t st)

@@ -251,12 +251,17 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         const double wa = Pa.w;
 
         // ---- 2. neighbour pairs
-        // flat pair index pi -> (u, v), u < v, row u holding v = u+1 .. K-1
-        int u = 0;
-        int rem = tid;
-        while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
+        // The triangle u < v is folded into rows of K: pair index pi = r*K + c is
+        // (u, v) = (r, r+1+c) for c < K-1-r, else (K-2-r, c) (row K-2-r's pairs).
+        // A lane's index advances by 256 per step, i.e. c by 256 mod K with at most
+        // one carry into r: no loop, and neighbouring lanes read neighbouring entries.
+        const int Kc = max(K, 1);
+        const int dq = kWG / Kc, dr = kWG - dq * Kc;
+        int r = tid / Kc, c = tid - r * Kc;
         for (int pi = tid; pi < P; pi += kWG) {
-            const int v = u + 1 + (int)rem;
+            const bool first = c < K - 1 - r;
+            const int u = first ? r : K - 2 - r;
+            const int v = first ? r + 1 + c : c;
             uint32_t pu, pv;
             int b3 = -1;
             if constexpr (FP64) {
@@ -331,8 +336,10 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     }
                 }
             }
-            rem += kWG;
-            while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
+            c += dr;
+            const bool carry = c >= Kc;
+            c -= carry ? Kc : 0;
+            r += dq + (carry ? 1 : 0);
         }
         bound += (unsigned long long)P;
         __syncthreads();

@@ -3,7 +3,7 @@
 pair index t and advancing by 256; pair index pi = r*K + c maps to
 (u, v) = (r, r+1+c) if c < K-1-r, else (K-2-r, c).  Every unordered pair
 u < v of K neighbours must be visited exactly once (each apex pair is one
-triangle candidate of the reference's k1 < k2 loop, gramsci.F90:478-481)."""
+triangle candidate of the reference's k1 < k2 loops, gramsci.F90:479-489)."""
 import pytest
 
 NTHR = 256

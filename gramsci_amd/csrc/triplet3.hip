@@ -51,7 +51,8 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
     return b;
 }
 
-// record packing: bits 0..7 bin (0-based), bit 8 random, bits 9..15 mu bin (0-based)
+// record packing: bits 0..13 bin * nb (0-based bin, pre-multiplied for the configuration
+// index), bit 14 random, bits 15..21 mu bin (0-based)
 //
 // The variant is a template parameter so that each one pays only for its own
 // registers: MODE bit 0 RSD (nm >= 2), bit 1 equilateral (0 = plain); GEN 1 for
@@ -217,11 +218,11 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const int pos = atomicAdd(&s_k, 1);
                 if (pos >= kmax) { s_over = 1; continue; }
                 const uint32_t fj = a.flags[j];
-                uint32_t pk = (uint32_t)b | ((fj & 1u) << 8);
+                uint32_t pk = __umul24((uint32_t)b, (uint32_t)nb) | ((fj & 1u) << 14);
                 if (nm > 1) {
                     int m, mr;
                     exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);   // apex -> j
-                    pk |= (uint32_t)m << 9;
+                    pk |= (uint32_t)m << 15;
                 }
                 if (nm > 1 || W) ol[pos] = a.orig[j];
                 if constexpr (FP64) {
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const double4 Ru = rec64[u], Rv = rec64[v];
                 pu = (uint32_t)Ru.w;
                 pv = (uint32_t)Rv.w;
-                if (!equi || ((pu ^ pv) & 255u) == 0) {
+                if (!equi || ((pu ^ pv) & 0x3fffu) == 0) {
                     const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
                     if (s > t_lo && s < t_hi) {
                         const float sf = (float)s;
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 pv = __float_as_uint(Rv.w);
                 const float dx = Rv.x - Ru.x, dy = Rv.y - Ru.y, dz = Rv.z - Ru.z;
                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if ((!equi || ((pu ^ pv) & 255u) == 0) && s > rej_lo && s < rej_hi) {
+                if ((!equi || ((pu ^ pv) & 0x3fffu) == 0) && s > rej_lo && s < rej_hi) {
                     const int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
                     const float2 ok = okb[b];
                     if ((s >= ok.x) & (s < ok.y)) {
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     }
                 }
             }
-            if (equi && b3 != (int)(pu & 255u)) b3 = -1;
+            if (equi && b3 >= 0 && __umul24((uint32_t)b3, (uint32_t)nb) != (pu & 0x3fffu)) b3 = -1;
             // the triple product in the reference's order, (w_i w_k1) w_k2 with k1 the neighbour of
             // smaller original index (gramsci.F90:482, :501), so that sums do not depend on the
             // order in which neighbours were gathered
@@ -309,13 +310,13 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 return (wa * ldexp(wl[xf ? x : y], -shift)) * wl[xf ? y : x];
             };
             if (b3 >= 0) {
-                const int cfg = equi ? b3 : (int)tab[__umul24(__umul24(pu & 255u, nb) + (pv & 255u), nb) + b3];
+                const int cfg = equi ? b3 : (int)tab[__umul24(pu & 0x3fffu, (uint32_t)nb) + (pv & 0x3fffu) + (uint32_t)b3];
                 int mu = 0;
                 if (nm > 1) {
-                    const int mu_u = (int)((pu >> 9) & 127u), mu_v = (int)((pv >> 9) & 127u);
+                    const int mu_u = (int)((pu >> 15) & 127u), mu_v = (int)((pv >> 15) & 127u);
                     mu = ol[u] < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
                 }
-                const int cls = ra + (int)((pu >> 8) & 1u) + (int)((pv >> 8) & 1u);
+                const int cls = ra + (int)((pu >> 14) & 1u) + (int)((pv >> 14) & 1u);
                 const int hk = (cfg * nm + mu) * 4 + cls;
                 ++binned;
                 if (lh) {

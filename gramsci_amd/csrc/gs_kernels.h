@@ -131,6 +131,11 @@ struct QuadArgs {
     int32_t kmax;
     int32_t lds_hist;           // counts in LDS (else global atomics)
     int32_t tetra;              // regular tetrahedra only (all six sides in one bin; cell = that bin)
+    // two-pass runs, as for the triplet kernel: with ovf set, an apex with more than kmax
+    // neighbours is queued (count in *ovf_n) and skipped; with items set, work item t is apex items[t]
+    int32_t* ovf;
+    int32_t* ovf_n;
+    const int32_t* items;
 };
 size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra);
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);

@@ -407,12 +407,78 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     a.kmax = kmax;
     a.lds_hist = lds_hist ? 1 : 0;
     a.tetra = tetra ? 1 : 0;
-    const int occ = quad4_occupancy(nb, ncell, kmax, lds_hist, tetra, weighted);
-    int nblocks = std::max(1, std::min(v.n_cu * occ, a.n_items));
+    // Two passes, as for the 3PCF: the first pass's neighbour list (and its K x K link matrix)
+    // is sized for the 99.5th percentile of ~8192 sampled neighbour counts (+5%, a power of two
+    // for the bitonic sort, at least 64 for the sort's staging area); apexes with more
+    // neighbours are queued for a second pass with the full list.
+    const int n = a.n_items;
+    int kmax1 = kmax;
+    const int nsample = std::min(n, 8192);
+    const int stride = nsample > 0 ? n / nsample : 1;
+    if ((rc = X.i32.reserve((size_t)n + 1 + nsample))) return rc;
+    if (nsample > 0 && kmax > 64) {
+        TripArgs ts{};
+        ts.pw = a.pw;
+        ts.flags = a.flags;
+        ts.cell_start = a.cell_start;
+        ts.rows = a.rows;
+        ts.bins = a.bins;
+        ts.g = a.g;
+        ts.nrows = a.nrows;
+        int32_t* d_cnt = X.i32.ptr + n + 1;
+        std::vector<int32_t> kc(nsample);
+        e = launch_nbr_count(ts, stride, nsample, d_cnt, v.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(kc.data(), d_cnt, nsample * sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+        if (e != hipSuccess) return hip_fail(e, "4PCF neighbour sample");
+        std::vector<int32_t> ks;
+        for (int32_t k : kc)
+            if (k >= 0) ks.push_back(k);
+        if (!ks.empty()) {
+            const size_t q = std::min(ks.size() - 1, (size_t)std::ceil(0.995 * (ks.size() - 1)));
+            std::nth_element(ks.begin(), ks.begin() + q, ks.end());
+            const double want = 1.05 * ks[q];
+            kmax1 = 64;
+            while (kmax1 < kmax && kmax1 < want) kmax1 *= 2;
+        }
+    }
+    {
+        const char* ev = getenv("GS_T4_KMAX");      // tuning override (unset or empty: automatic)
+        if (ev && *ev) {
+            kmax1 = 64;
+            while (kmax1 < kmax && kmax1 < atoi(ev)) kmax1 *= 2;
+        }
+    }
+    const bool two_pass = kmax1 < kmax;
     (void)hipEventRecord(v.ev[2], v.stream);
-    if (a.n_items > 0) {
-        e = launch_quad4(a, nb, weighted, nblocks, v.stream);
+    if (n > 0) {
+        QuadArgs a1 = a;
+        if (two_pass) {
+            a1.kmax = kmax1;
+            a1.ovf_n = X.i32.ptr;
+            a1.ovf = X.i32.ptr + 1;
+            e = hipMemsetAsync(X.i32.ptr, 0, sizeof(int32_t), v.stream);
+            if (e != hipSuccess) return hip_fail(e, "4PCF overflow counter");
+        }
+        const int occ = quad4_occupancy(nb, ncell, a1.kmax, lds_hist, tetra, weighted);
+        e = launch_quad4(a1, nb, weighted, std::max(1, std::min(v.n_cu * occ, n)), v.stream);
         if (e != hipSuccess) return hip_fail(e, "4PCF kernel launch");
+        if (two_pass) {
+            int32_t nov = 0;
+            e = hipMemcpyAsync(&nov, X.i32.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+            if (e != hipSuccess) return hip_fail(e, "4PCF first pass");
+            if (nov > 0) {
+                QuadArgs a2 = a;
+                a2.items = X.i32.ptr + 1;
+                a2.n_items = nov;
+                e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);   // work counters only
+                const int occ2 = quad4_occupancy(nb, ncell, kmax, lds_hist, tetra, weighted);
+                if (e == hipSuccess)
+                    e = launch_quad4(a2, nb, weighted, std::max(1, std::min(v.n_cu * occ2, nov)), v.stream);
+                if (e != hipSuccess) return hip_fail(e, "4PCF second pass");
+            }
+        }
     }
     (void)hipEventRecord(v.ev[3], v.stream);
     std::vector<unsigned long long> h_out(nout);

@@ -94,8 +94,9 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             s_over = 0;
         }
         __syncthreads();
-        const int apex = s_item;
-        if (apex < 0) break;
+        const int item = s_item;
+        if (item < 0) break;
+        const int apex = a.items ? a.items[item] : item;
         const uint32_t fa = a.flags[apex];
         if (fa & 2u) {
             __syncthreads();
@@ -130,7 +131,15 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         }
         __syncthreads();
         const int K = min(s_k, kmax);
-        if (s_over && tid == 0) atomicExch(a.err, 1);
+        if (s_over) {
+            if (a.ovf) {
+                // left to the second pass, which has room for all its neighbours
+                if (tid == 0) a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
+                __syncthreads();
+                continue;
+            }
+            if (tid == 0) atomicExch(a.err, 1);
+        }
         if (K <= (tetra ? 2 : 3)) {        // gramsci.F90:675 (the tetrahedron query has no such skip)
             __syncthreads();
             continue;

@@ -174,6 +174,29 @@ def test_4pcf_weighted_buffer_and_order(eng):
     check4(eng, xyz[perm], None, 3000, O.Params(3.0, 9.0, 3, 1, False))
 
 
+@pytest.mark.parametrize("tetra", [False, True])
+def test_4pcf_two_pass_overflow(eng, monkeypatch, tetra):
+    # a 64-entry first-pass list: apexes with more neighbours (about half here) are counted by
+    # the second pass; counts equal the oracle's and the single-pass run's
+    d = C.clustered(3000, 60.0, 97)
+    r = C.uniform(3000, 60.0, 98)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(3000, 0.2, 99), np.ones(3000)])
+    p = O.Params(2.0, 7.0, 3, 1, False)
+    eng.set_catalogue(xyz, w, 3000)
+    fn = eng.pcf4_tetra if tetra else eng.pcf4
+    base = fn(p.rmin, p.rmax, p.nbins, p.log_bins)
+    monkeypatch.setenv("GS_T4_KMAX", "64")
+    two = fn(p.rmin, p.rmax, p.nbins, p.log_bins)
+    assert np.array_equal(two["cnt"], base["cnt"])
+    assert two["stats"]["n_units"] == base["stats"]["n_units"]
+    assert _close(two["n2"], base["n2"]) and _close(two["n3"], base["n3"])
+    if not tetra:
+        ref = O.pcf4(xyz, w, 3000, p)
+        assert np.array_equal(two["cnt"], ref["cnt"]) and two["stats"]["n_units"] == ref["n_quads"]
+        assert _close(two["disc_n"], ref["disc_n"]) and _close(two["disc_r"], ref["disc_r"])
+
+
 def test_4pcf_lattice_and_log(eng):
     lat = C.lattice(9, 1.0, 0.0)
     check4(eng, lat, None, len(lat), O.Params(0.5, 2.5, 2, 1, False))

@@ -107,6 +107,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), nyg - 1);
         const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nzg - 1);
 
+        const unsigned long long cand0 = cand, over0 = over;   // restored if the apex is queued
         // ---- 1. gather
         for (int r = 0; r < a.nrows; ++r) {
             const Row rw = a.rows[r];
@@ -135,6 +136,8 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             if (a.ovf) {
                 // left to the second pass, which has room for all its neighbours
                 if (tid == 0) a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
+                cand = cand0;       // the second pass counts this apex's candidates
+                over = over0;
                 __syncthreads();
                 continue;
             }

@@ -196,6 +196,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         const int cy = min(max((int)floor((Pa.y - ly) * inv_h), 0), ny - 1);
         const int cz = min(max((int)floor((Pa.z - lz) * inv_h), 0), nz - 1);
 
+        const unsigned long long cand0 = cand, over0 = over;   // restored if the apex is queued
         // ---- 1. gather the kept neighbours with their exact bin
         for (int r = 0; r < a.nrows; ++r) {
             const Row rw = a.rows[r];
@@ -241,6 +242,8 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             if (a.ovf) {
                 // left to the second pass, which has room for all its neighbours
                 if (tid == 0) a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
+                cand = cand0;       // the second pass counts this apex's candidates
+                over = over0;
                 __syncthreads();
                 continue;
             }

@@ -150,6 +150,9 @@ def test_3pcf_two_pass_overflow(eng, monkeypatch, nmu):
     for o in outs:
         assert np.array_equal(o["cnt"], base["cnt"]) and np.array_equal(o["wsum"], base["wsum"])
         assert o["stats"]["n_units"] == base["stats"]["n_units"]
+        # every apex's candidates and overflow pairs are counted in exactly one pass
+        assert o["stats"]["n_candidates"] == base["stats"]["n_candidates"]
+        assert o["stats"]["n_overflow"] == base["stats"]["n_overflow"]
 
 
 def test_3pcf_config3_density_slab(eng):
@@ -190,6 +193,7 @@ def test_4pcf_two_pass_overflow(eng, monkeypatch, tetra):
     two = fn(p.rmin, p.rmax, p.nbins, p.log_bins)
     assert np.array_equal(two["cnt"], base["cnt"])
     assert two["stats"]["n_units"] == base["stats"]["n_units"]
+    assert two["stats"]["n_candidates"] == base["stats"]["n_candidates"]
     assert _close(two["n2"], base["n2"]) and _close(two["n3"], base["n3"])
     if not tetra:
         ref = O.pcf4(xyz, w, 3000, p)

@@ -73,6 +73,9 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
     const float ms32 = B.ms32;
     const float dt_ew = ms32 * B.mu_ew, dt_em = ms32 * B.mu_em, dt_c = ms32 * B.mu_c;   // mu error bound
     const float nbm1f = (float)(nb - 1), nmm1f = (float)(nm - 1);
+    // mu edges are the integers 1..nm-1 of t; with nm = 127 also 0 and 127, where the reference's
+    // int8 store of floor(t)+1 wraps (gramsci.F90:326-328), so pairs there get the FP64 recheck
+    const float e_lo = nm == 127 ? 0.0f : 1.0f, e_hi = nm == 127 ? 127.0f : nmm1f;
     const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
     const int nrow = ny * nz;
     const float h32 = a.g.h32;
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
                             const float t = fmaf(D * rm * rsq, ms32, ms32);       // (theta + 1) * mu_scale
                             const float dt = fmaf(dt_ew, rsq, fmaf(dt_em, rm, dt_c));
                             // certain when the nearest mu edge (an integer 1..nm-1 of t) is farther than dt
-                            const float edge = __builtin_amdgcn_fmed3f(rintf(t), 1.0f, nmm1f);
+                            const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                             safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
                             m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                             mr = nm - 1 - m;

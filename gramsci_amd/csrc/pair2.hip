@@ -211,6 +211,9 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     const int nbm = nb * nm;
     const float ms32 = B.ms32, mu_ew = B.mu_ew, mu_em = B.mu_em, mu_c = B.mu_c;
     const float nbm1f = (float)(nb - 1), nmm1f = (float)(nm - 1);
+    // mu edges are the integers 1..nm-1 of t; with nm = 127 also 0 and 127, where the reference's
+    // int8 store of floor(t)+1 wraps (gramsci.F90:326-328), so pairs there get the FP64 recheck
+    const float e_lo = nm == 127 ? 0.0f : 1.0f, e_hi = nm == 127 ? 127.0f : nmm1f;
     // mu error bound dt = ms * (mu_ew / |sep| + mu_em / |mid| + mu_c), with ms folded into the
     // constants (their 4x margin dwarfs the reassociation).  In the weighted kernel, whose
     // residency LDS limits, dt_c and ub are kept in VGPRs so that each FMA of the loop reads at
@@ -420,7 +423,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
                                     // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
                                     // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
-                                    const float edge = __builtin_amdgcn_fmed3f(rintf(t), 1.0f, nmm1f);
+                                    const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                                     safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
                                     m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                                     mr = nm - 1 - m;
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
                                 // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
                                 // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
-                                const float edge = __builtin_amdgcn_fmed3f(rintf(t), 1.0f, nmm1f);
+                                const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                                 safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
                                 m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
                                 mr = nm - 1 - m;

@@ -212,3 +212,16 @@ def test_repeat_is_deterministic(eng):
     b = eng.pcf2(1.0, 30.0, 10, 6, True, "fp32", cell_div=2)
     assert np.array_equal(a["cnt"], b["cnt"])
     assert np.array_equal(a["wsum"], b["wsum"])   # fixed point: order independent
+
+
+def test_mu_int8_wrap_at_127_mu_bins(eng):
+    # pairs on rays from the origin have theta = +-1; at nmu = 127 the reference's int8 store of
+    # floor(t)+1 wraps and sends them to mu bin 1 (the oracle follows it; pinned against the
+    # reference binary in test_oracle.py), which the FP32 guard must hand to the FP64 recheck
+    from test_oracle import ray_catalogue
+    xyz = ray_catalogue()
+    nd = len(xyz) // 2
+    w = C.lognormal_weights(len(xyz), 0.2, 6)
+    for wt in (None, w):
+        ref, outs = _check(eng, xyz, wt, nd, O.Params(0.5, 12.0, 4, 127, False))
+        assert ref["cnt"][0][:, 0].sum() > 0

@@ -14,6 +14,7 @@ and, independently of the reference:
 """
 import itertools
 import math
+import os
 
 import numpy as np
 import pytest
@@ -352,3 +353,26 @@ def test_invalid_params_rejected():
         O.pcf2(xyz, None, 4, O.Params(0.0, 1.0, 3, log_bins=True))
     with pytest.raises(ValueError):
         O.pcf2(xyz, None, 4, O.Params(0.0, 1.0, 200))
+
+
+def ray_catalogue(seed=5):
+    """Points on rays from the origin (pairs with theta = +-1 exactly on the axes) mixed with
+    uniform points; at nmu = 127 the reference's int8 store of floor(t)+1 wraps for theta >= 1."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(1, 161, dtype=float)
+    rays = [np.stack([0.5 * k, 0 * k, 0 * k], 1), np.stack([0 * k, 0.75 * k, 0 * k], 1),
+            np.stack([0 * k, 0 * k, -0.625 * k], 1), np.stack([0.5 * k, 0.5 * k, 0 * k], 1),
+            np.stack([-0.25 * k, 0.25 * k, 0.25 * k], 1)]
+    return np.concatenate(rays + [rng.uniform(-30.0, 30.0, (400, 3))])
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_BIN), reason="reference not built (make -C oracle ref)")
+def test_mu_int8_wrap_matches_reference():
+    xyz = ray_catalogue()
+    p = O.Params(0.5, 12.0, 4, 127, False)
+    res = O.pcf2(xyz, None, len(xyz), p)
+    rows = parse_rows(O.reference_table(xyz, None, None, None, p))
+    got = np.rint(rows[:, 4] * float(len(xyz)) ** 2).astype(np.int64).reshape(p.nbins, p.nmu)
+    assert np.array_equal(got, res["cnt"][0].astype(np.int64))
+    # the wrap is exercised: theta = 1 pairs land in mu bin 1, not bin 127
+    assert res["cnt"][0][:, 0].sum() > res["cnt"][0][:, -1].sum() > 0

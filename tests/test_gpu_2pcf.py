@@ -225,3 +225,15 @@ def test_mu_int8_wrap_at_127_mu_bins(eng):
     for wt in (None, w):
         ref, outs = _check(eng, xyz, wt, nd, O.Params(0.5, 12.0, 4, 127, False))
         assert ref["cnt"][0][:, 0].sum() > 0
+
+
+@pytest.mark.parametrize("shift", [5.0e4, -2.0e5, -60.0])
+def test_catalogue_far_from_or_around_origin(eng, shift):
+    # (r, mu) counts with the line of sight from the origin: a box 50000 / 200000 units away (large
+    # absolute coordinates, small mu error bound) and one straddling the origin (|mid| near 0)
+    d = C.clustered(4000, 120.0, 31) + shift
+    r = C.uniform(8000, 120.0, 32) + shift
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(4000, 0.2, 33), np.ones(8000)])
+    ref, outs = _check(eng, xyz, w, 4000, O.Params(1.0, 25.0, 8, 12, True))
+    assert ref["cnt"].sum() > 0

@@ -35,7 +35,7 @@ METRIC = "binned pair/triplet evals/s on 1 MI355X: batched FP32+guard vs plain F
 # HBM bytes per dispatch of the FP32 + guard pair kernel on the (r, mu) workload, measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) and scaled by the same counters'
 # readings for a copy of known size (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE)
-PAIR_RMU_TRAFFIC_BYTES = (34_946_654.1 * 2.0 + 121_075.4 * 1.0) * 1024
+PAIR_RMU_TRAFFIC_BYTES = (34_722_358.6 * 2.0 + 121_081.7 * 1.0) * 1024
 FLOPS_PER_PAIR_RMU = 22      # 3 sub, 3 mul, 2 add for s; 3 mid adds, m.s (5), |m|^2 (5), 1 product
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector, dense
 PEAK_FP64 = 78.6e12
@@ -306,7 +306,7 @@ def bench_c3(args):
     line["roofline"] = {"bound": "valu", "achieved": U * 8 / (out["stats"]["ms_count"] * 1e-3) / 1e12,
                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                         "frac": U * 8 / (out["stats"]["ms_count"] * 1e-3) / PEAK_FP32, "traffic": None,
-                        "kernel": "k_triplet3<false, false, 0, false>", "kernel_ms": out["stats"]["ms_count"],
+                        "kernel": "k_triplet3<false, false, 0, false, false>", "kernel_ms": out["stats"]["ms_count"],
                         "flops_per_unit": 8}
     if "fp64" in res:
         line["fp64"] = {"value": res["fp64"][1]["stats"]["n_units"] / res["fp64"][0],

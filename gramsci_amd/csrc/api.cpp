@@ -234,14 +234,24 @@ int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc
     B.mu_ew = (float)(2.0 * std::sqrt(3.0) * ew * 4.0);
     B.mu_em = (float)(2.0 * std::sqrt(3.0) * em * 4.0);
     B.mu_c = (float)std::ldexp(1.0, -18);
-    // fixed point: the largest pair product stays below 2^58 (the pair kernel sums runs of up
-    // to 33 of them in one 64-bit register), the largest triple product below 2^61
-    const double wm = std::max(wmax, 1e-300);
-    B.fx_shift2 = 58 - (int)std::ceil(std::log2(wm * wm * (1.0 + 1e-12)));
-    B.fx_shift3 = 61 - (int)std::ceil(std::log2(wm * wm * wm * (1.0 + 1e-12)));
-    if (B.fx_shift2 > 1000) B.fx_shift2 = 1000;
-    if (B.fx_shift3 > 1000) B.fx_shift3 = 1000;
+    B.fx_shift2 = fx_shift_pairs(wmax);
+    B.fx_shift3 = fx_shift_triples(wmax);
     return GS_OK;
+}
+
+// Fixed-point scales 2^F of the weight products: the largest pair product stays below 2^52
+// (the pair kernel rounds it to an integer with one FP64 addition of 2^52 and sums runs of up
+// to 33 in one 64-bit register), the largest triple product below 2^61.
+int fx_shift_pairs(double wmax)
+{
+    const double wm = std::max(wmax, 1e-300);
+    return std::min(1000, 52 - (int)std::ceil(std::log2(wm * wm * (1.0 + 1e-12))));
+}
+
+int fx_shift_triples(double wmax)
+{
+    const double wm = std::max(wmax, 1e-300);
+    return std::min(1000, 61 - (int)std::ceil(std::log2(wm * wm * wm * (1.0 + 1e-12))));
 }
 
 // Half stencil of x-rows for cells of edge h and a search radius rmax, m = ceil(rmax/h).

@@ -91,6 +91,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         d.row_base = (int32_t)rows_tot;
         d.pt_base = (int32_t)B.off[r];
         d.n_data = (int32_t)B.ndata[r];
+        // the scale a launch over this region alone would use, so that sums match it bit for bit
+        d.fx_shift = kind == 3 ? fx_shift_triples(B.wmr[r]) : fx_shift_pairs(B.wmr[r]);
         row_base[r] = (int32_t)rows_tot;
         cells += nc;
         rows_tot += (int64_t)d.dim[1] * d.dim[2];
@@ -288,6 +290,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     P.exec = exec;
     P.nbt = nbt;
     P.fx_shift = kind == 3 ? bins.fx_shift3 : bins.fx_shift2;
+    P.fx_shift_r.resize(nreg);
+    for (int r = 0; r < nreg; ++r) P.fx_shift_r[r] = reg[r].fx_shift;
     P.mst = mst;
     P.kind = kind;
     P.nh = nh;
@@ -331,6 +335,7 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
     B.bbox.assign((size_t)6 * nreg, 0.0);
     B.wd.assign(nreg, 0.0L);
     B.wr.assign(nreg, 0.0L);
+    B.wmr.assign(nreg, 1.0);
     B.wmax = 1.0;
     B.xmax = 0.0;
     double wm = 0.0;
@@ -355,6 +360,7 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
         }
         // the region's sums exactly as a single-catalogue upload of the region computes them
         if (w) {
+            B.wmr[r] = wmr > 0 ? wmr : 1.0;
             const int k = wsum_scale(wmr);
             B.wd[r] = fixed_sum(w + off[r], ndata[r], k);
             B.wr[r] = fixed_sum(w + off[r] + ndata[r], off[r + 1] - off[r] - ndata[r], k);
@@ -431,7 +437,7 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         for (int k = 0; k < nbt; ++k) {
             if (cnt) cnt[(size_t)r * nbt + k] = hc[k];
             total += hc[k];
-            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift) : (long double)hc[k];
+            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift_r[r]) : (long double)hc[k];
             if (wsum) wsum[(size_t)r * nbt + k] = (double)S[k];
         }
         const long double Wd = B.wd[r], Wr = B.wr[r];
@@ -509,7 +515,7 @@ extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         const unsigned long long* hw = h.data() + (size_t)nreg * nh + (size_t)r * 2 * nh;
         for (int k = 0; k < nh; ++k) {
             if (cnt) cnt[(size_t)r * nh + k] = hc[k];
-            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift) : (long double)hc[k];
+            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift_r[r]) : (long double)hc[k];
             if (wsum) wsum[(size_t)r * nh + k] = (double)S[k];
         }
         combine3(S.data(), ncfg * nm, B.wd[r], B.wr[r], true, n2 ? n2 + (size_t)r * ncfg * nm : nullptr,

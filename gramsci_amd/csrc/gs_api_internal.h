@@ -41,6 +41,7 @@ struct GsBatch {
     std::vector<int64_t> off, ndata;
     std::vector<double> bbox;            // [nreg][6]
     std::vector<long double> wd, wr;     // per-region weight sums
+    std::vector<double> wmr;             // per-region largest weight (1 for unit weights)
     double wmax = 1.0, xmax = 0.0;
     DevBuf<int32_t> d_off, d_row_base, cell_of, count, cell_start, nch, row_chunk, scan_tmp, queue, orig;
     DevBuf<int32_t> ovf;                 // 3PCF: overflow count + apexes left to the second pass
@@ -59,6 +60,7 @@ struct GsBatch {
         hipGraphExec_t exec = nullptr;
         int nbt = 0;
         int fx_shift = 0;
+        std::vector<int> fx_shift_r;     // per-region fixed-point scale (weighted runs)
         int mst = 0;
         int kind = 2;                    // 2: pair kernel, 3: triplet kernel
         int nh = 0;                      // triplet histogram entries per region
@@ -101,6 +103,9 @@ double round11(double h);
 int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
 int check_params(const gs_params* p, int mode);
 int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B);
+// fixed-point scales (exponents F of 2^F) of pair and triple weight products for a largest weight wmax
+int fx_shift_pairs(double wmax);
+int fx_shift_triples(double wmax);
 // sum of w[0..n) as the exact sum of the fixed-point integers wsum_fixed(w[i], k), as a long double
 long double fixed_sum(const double* w, int64_t n, int k);
 std::vector<Row> half_stencil(double h, double rmax, int m);

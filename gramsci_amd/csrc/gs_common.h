@@ -53,7 +53,7 @@ struct RegionDesc {
     int32_t row_base;        // first global x-row of the region
     int32_t pt_base;         // first point (input order and cell-sorted order)
     int32_t n_data;          // data points of the region (the rest are randoms)
-    int32_t pad;
+    int32_t fx_shift;        // the region's fixed-point scale of weight products (as its own launch)
 };
 
 // Exact, order-independent weight sums (the normalisations sum w over data

@@ -96,12 +96,12 @@ __device__ __forceinline__ void quantise(double x, uint32_t& lo, uint32_t& hi)
     lo = (uint32_t)fma(-h, 0x1p32, x);
 }
 
-// floor(x) as one 64-bit integer (the register pair lo, hi of quantise)
-__device__ __forceinline__ unsigned long long quantise64(double x)
+// rint(x) (round to nearest even) as a 64-bit integer, for 0 <= x < 2^52: x + 2^52
+// lies in [2^52, 2^53], where the spacing of doubles is 1, so the addition rounds x to an
+// integer and the bit pattern of the sum is that of 2^52 plus it (also at 2^53)
+__device__ __forceinline__ unsigned long long quantise_rn(double x)
 {
-    uint32_t lo, hi;
-    quantise(x, lo, hi);
-    return ((unsigned long long)hi << 32) | lo;
+    return (unsigned long long)(__double_as_longlong(x + 0x1p52) - 0x4330000000000000ll);
 }
 
 // largest r with off[r] <= v over off[0..n-1] (off non-decreasing)

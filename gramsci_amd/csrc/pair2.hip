@@ -77,7 +77,11 @@ __device__ __forceinline__ int accumulate(uint32_t* hc, unsigned long long* hw, 
                                           double x, unsigned long long* out_cnt, unsigned long long* out_w)
 {
     uint32_t qlo = 0, qhi = 0;
-    if constexpr (W) quantise(x, qlo, qhi);
+    if constexpr (W) {
+        const unsigned long long q = quantise_rn(x);
+        qlo = (uint32_t)q;
+        qhi = (uint32_t)(q >> 32);
+    }
     auto add = [&](int k, uint32_t c) {
         atomicAdd(&hc[(k << rsh) + rep], c);
         if constexpr (W) {
@@ -221,7 +225,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     const float dt_ew = ms32 * mu_ew, dt_em = ms32 * mu_em;
     float dt_c = ms32 * mu_c, ubv = ub;
     if constexpr (W) asm volatile("" : "+v"(dt_c), "+v"(ubv));
-    const int shift = B.fx_shift2;
+    int shift = B.fx_shift2;         // batched: the current region's scale
     const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
     const int n_chunks = batched ? *a.n_chunks_dev : a.n_chunks;
     const float h32 = a.g.h32;
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     // neighbours (adjacent points of one cell, data before randoms) often fall in the same
     // bin; their updates are merged in registers and reach the LDS histogram with one set of
     // atomics when the bin changes, which relieves same-address LDS atomic conflicts.
-    // The run's sum is one 64-bit integer: products are below 2^58 (fx_shift2) and a run ends
+    // The run's sum is one 64-bit integer: products are below 2^52 (fx_shift2) and a run ends
     // after at most 33 increments, so it cannot overflow; the flush splits it at bit 32 into
     // the two LDS halves, which keeps value = hi * 2^32 + lo.  The update is written with
     // selects, not branches, so that no register copies are needed where paths merge.
@@ -278,6 +282,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
             nx = rd.dim[0]; ny = rd.dim[1]; nz = rd.dim[2];
             cb = rd.cell_base;
             row = grow - rd.row_base;
+            shift = rd.fx_shift;
             if (R != cur_r) {
                 if constexpr (W) flush();
                 if (cur_r >= 0)
@@ -446,7 +451,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     const int cj = (ep & 2) ? 0 : 1;
                                     const int base = ((ep & 1) ? cb1 : cb0) + __umul24(b, nm);
                                     unsigned long long q = 0;
-                                    if constexpr (W) q = quantise64(wi * ewj);
+                                    if constexpr (W) q = quantise_rn(wi * ewj);
                                     if constexpr (W && !RSD) {
                                         if (ci + cj) put(base, (uint32_t)(ci + cj), q);
                                     } else if (W && mirror && mirrored) {

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const float nbm1f = (float)(nb - 1);
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
-    const int shift = B.fx_shift3;
+    int shift = B.fx_shift3;         // batched: the apex's region's scale
     const double inv_h = 1.0 / a.g.h;
     const bool batched = GEN && a.nreg > 0;
     int cur_r = batched ? -1 : 0;   // region whose histogram the LDS holds (work-group uniform)
@@ -181,6 +181,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             const RegionDesc& rd = a.regions[lo];
             nx = rd.dim[0]; ny = rd.dim[1]; nz = rd.dim[2];
             cb = rd.cell_base;
+            shift = rd.fx_shift;
             lx = rd.lo[0]; ly = rd.lo[1]; lz = rd.lo[2];
             if (lo != cur_r) {
                 if (lh && cur_r >= 0) {

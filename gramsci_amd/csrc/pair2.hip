@@ -384,9 +384,10 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                     if constexpr (W) {
                         // Register tile: lane L loads entry L; the entries then rotate through the
                         // wave one lane per step (DPP), so after 64 steps every lane has met every
-                        // entry (after P steps for a tile of nt <= P entries, below).  Consecutive lanes meet different neighbours at each step, which
-                        // spreads their histogram updates over many bins, and no LDS is read for
-                        // the neighbours.  Used for weighted sums, where it beats the LDS tile.
+                        // entry (after P steps for a tile of nt <= P entries, below).  Consecutive
+                        // lanes meet different neighbours at each step, which spreads their histogram
+                        // updates over many bins, and no LDS is read for the neighbours.  Used for
+                        // weighted sums, where it beats the LDS tile.
                         // A partial tile is loaded with period P (the power of two >= nt): lane L
                         // holds entry L mod P, a full-wave rotation keeps that pattern, and P steps
                         // bring every entry to every lane.

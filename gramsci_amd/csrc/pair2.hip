@@ -53,12 +53,11 @@ struct Stats {
 };
 
 // Rotation of a register across the 64 lanes of a wavefront by one lane (DPP
-// wave_ror:1); every lane must be active.
+// wave_rol:1); every lane must be active.
 // The register itself is passed as the unused "old" operand (a full rotation
 // writes every lane), so the move can be done in place: one v_mov_b32_dpp per
 // register and step, with no zeroed temporary and no copy back.
 __device__ __forceinline__ int rot_lane(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xf, 0xf, false); }
-__device__ __forceinline__ uint32_t rot_lane(uint32_t v) { return (uint32_t)rot_lane((int)v); }
 __device__ __forceinline__ float rot_lane(float v) { return __int_as_float(rot_lane(__float_as_int(v))); }
 __device__ __forceinline__ double rot_lane(double v)
 {

@@ -35,7 +35,7 @@ METRIC = "binned pair/triplet evals/s on 1 MI355X: batched FP32+guard vs plain F
 # HBM bytes per dispatch of the FP32 + guard pair kernel on the (r, mu) workload, measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) and scaled by the same counters'
 # readings for a copy of known size (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE)
-PAIR_RMU_TRAFFIC_BYTES = (34_722_358.6 * 2.0 + 121_081.7 * 1.0) * 1024
+PAIR_RMU_TRAFFIC_BYTES = (34_301_089.1 * 2.0 + 103_883.1 * 1.0) * 1024
 FLOPS_PER_PAIR_RMU = 22      # 3 sub, 3 mul, 2 add for s; 3 mid adds, m.s (5), |m|^2 (5), 1 product
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector, dense
 PEAK_FP64 = 78.6e12

@@ -36,6 +36,9 @@ METRIC = "binned pair/triplet evals/s on 1 MI355X: batched FP32+guard vs plain F
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) and scaled by the same counters'
 # readings for a copy of known size (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE)
 PAIR_RMU_TRAFFIC_BYTES = (34_301_089.1 * 2.0 + 103_883.1 * 1.0) * 1024
+# VALU wave-instructions per dispatch of the same kernel on the same workload (rocprofv3 --pmc SQ_INSTS_VALU);
+# divided by the kernel time it is the issue rate, read against the rate of a pure FP32 FMA stream
+PAIR_RMU_VALU_WAVE_INSTR = 183_368_978_373
 FLOPS_PER_PAIR_RMU = 22      # 3 sub, 3 mul, 2 add for s; 3 mid adds, m.s (5), |m|^2 (5), 1 product
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector, dense
 PEAK_FP64 = 78.6e12
@@ -225,6 +228,15 @@ def bench_rmu(args):
                         "frac": achieved / peak, "traffic": PAIR_RMU_TRAFFIC_BYTES if main_prec == "fp32" else None,
                         "kernel": f"k_pair2<{'true' if main_prec == 'fp64' else 'false'}, true, true, true>",
                         "kernel_ms": R["t_kern"], "flops_per_pair": FLOPS_PER_PAIR_RMU}
+    if main_prec == "fp32":
+        # instruction-issue reading of the same launch: its VALU wave-instructions (counter) per second
+        # against those of a pure FP32 FMA stream timed now on the same GPU
+        c = valu_ceiling(3, 1)
+        rate = PAIR_RMU_VALU_WAVE_INSTR / (R["t_kern"] * 1e-3)
+        line["roofline"]["issue"] = {"valu_wave_instr": PAIR_RMU_VALU_WAVE_INSTR, "rate": rate,
+                                     "ceiling": c["wave_instr_per_s"], "frac": rate / c["wave_instr_per_s"],
+                                     "unit": "VALU wave-instructions/s",
+                                     "ceiling_kernel": "k_valu_fma32 (FP32 FMA stream, %.1f TFLOP/s)" % c["tflops"]}
     if "fp64" in res and main_prec != "fp64":
         F = res["fp64"]
         line["fp64"] = {"value": F["out"]["stats"]["n_binned"] / F["t_step"], "ms_per_step": F["t_step"] * 1e3,
@@ -495,7 +507,52 @@ def bench_copy(args):
                      "copy of 2^27 float64 (1 GiB read + 1 GiB written)", bytes_read=8 * n, bytes_written=8 * n)
 
 
-WORKLOADS = {"copy": bench_copy, "rmu": bench_rmu, "c1": bench_c1, "c3": bench_c3, "c3_4pcf": bench_c3_4pcf, "c4": bench_c4,
+def valu_ceiling(steps, warmup):
+    """Times the FP32 FMA issue-rate calibration kernel (gramsci_amd/libgs_calib.so: 8 independent FMA
+    chains per lane, 8 work-groups of 256 threads per CU, no memory traffic). Returns a dict with the mean
+    launch time (s), the VALU wave-instructions per launch and the resulting rates."""
+    import ctypes
+    import torch
+    so = os.path.join(ROOT, "gramsci_amd", "libgs_calib.so")
+    if not os.path.exists(so):
+        raise RuntimeError(f"{so} is not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(so)
+    lib.gs_calib_fma32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.POINTER(ctypes.c_float)]
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    threads, per_iter = lib.gs_calib_threads(), lib.gs_calib_fma_per_iter()
+    blocks, iters = n_cu * 8 * 4, 4096
+    out = torch.zeros(blocks * threads, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ms = ctypes.c_float(0.0)
+    times = []
+    for k in range(warmup + steps):
+        rc = lib.gs_calib_fma32(blocks, iters, ctypes.c_void_p(out.data_ptr()), out.numel(), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"gs_calib_fma32 failed: {rc}")
+        if k >= warmup:
+            times.append(ms.value * 1e-3)
+    dt = sum(times) / len(times)
+    wave_instr = blocks * (threads // 64) * iters * per_iter
+    return {"seconds": dt, "wave_instr": wave_instr, "wave_instr_per_s": wave_instr / dt,
+            "tflops": 2 * 64 * wave_instr / dt / 1e12, "n_cu": n_cu, "blocks": blocks, "iters": iters,
+            "per_iter": per_iter}
+
+
+def bench_valu(args):
+    """FP32 FMA issue-rate calibration: the practical VALU ceiling that the counting kernels'
+    instruction issue is read against."""
+    c = valu_ceiling(args.steps, args.warmup)
+    line = line_base("FP32 FMA issue rate (VALU calibration)", "TFLOP/s", c["tflops"], args.steps, args.warmup,
+                     c["seconds"], "f32", f"{c['blocks']} work-groups x 256 threads, {c['iters']} x {c['per_iter']} "
+                     "dependent-chain FMAs per thread (8 chains), packed FP32 off", n_cu=c["n_cu"])
+    line["valu_wave_instr_per_launch"] = c["wave_instr"]
+    line["valu_wave_instr_per_s"] = c["wave_instr_per_s"]
+    line["data"] = "none (register-only kernel)"
+    return line
+
+
+WORKLOADS = {"copy": bench_copy, "valu": bench_valu, "rmu": bench_rmu, "c1": bench_c1, "c3": bench_c3, "c3_4pcf": bench_c3_4pcf, "c4": bench_c4,
              "c5": bench_c5}
 
 

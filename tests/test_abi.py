@@ -20,6 +20,18 @@ def test_library_exports_every_declared_symbol():
     assert lib.gs_version().startswith(b"gramsci_hip")
 
 
+def test_calibration_library_loads():
+    # the VALU issue-rate calibration kernel that bench.py times beside the pair kernel
+    import os
+    so = os.path.join(os.path.dirname(L.__file__), "libgs_calib.so")
+    lib = ctypes.CDLL(so)
+    assert lib.gs_calib_threads() == 256 and lib.gs_calib_fma_per_iter() == 128
+    f = ctypes.c_float(0.0)
+    # argument checks refuse before any GPU call: an output buffer smaller than the grid
+    assert lib.gs_calib_fma32(4, 1, ctypes.c_void_p(1), ctypes.c_int64(1023), ctypes.byref(f)) == -1
+    assert lib.gs_calib_fma32(0, 1, ctypes.c_void_p(1), ctypes.c_int64(1 << 20), ctypes.byref(f)) == -1
+
+
 def _bin_from_thresholds(t, s):
     nb = len(t) - 2
     if not (s > t[0] and s < t[nb + 1]):

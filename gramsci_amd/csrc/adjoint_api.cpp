@@ -30,24 +30,48 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
     (void)hipEventRecord(v.ev[1], v.stream);
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = full_stencil(h, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, (mst + 1) * h, ctx_xmax(c), v.wmax, B))) return rc;
     if (!(v.wsum_d > 0) && v.n_data > 0) return fail(GS_EINVAL, "data weights sum to zero");
     if (!(v.wsum_r > 0) && v.n_rand > 0) return fail(GS_EINVAL, "random weights sum to zero");
     const double inv_wd = v.n_data > 0 ? (double)(1.0L / v.wsum_d) : 0.0;
     const double inv_wr = v.n_rand > 0 ? (double)(1.0L / v.wsum_r) : 0.0;
+    // FP32: the half-stencil kernel with exact fixed-point sums at the scale 2^F, chosen so that no pair
+    // term c * wn exceeds 2^50 (|c| <= 2 gmax, |wn| <= wmax / min(Wd, Wr)); split into a scale of the
+    // dL/dN tables (2^Fa, bringing gmax to [1, 2)) and one of the weights (2^Fb).  Non-finite dL/dN
+    // (or scales out of range) take the full-stencil kernel, which sums in FP64.
+    bool half = !fp64;
+    int F = 0, Fa = 0, Fb = 0;
+    {
+        double gmax = 0.0;
+        for (int k = 0; k < nb * nm; ++k) {
+            const double a2 = gN2[k], a23 = gN2[k] + gN3[k];
+            if (!std::isfinite(a2) || !std::isfinite(a23)) half = false;
+            else gmax = std::max(gmax, std::max(std::fabs(a2), std::fabs(a23)));
+        }
+        const double wn_max = v.wmax * std::max(inv_wd, inv_wr);
+        if (half && gmax > 0.0 && wn_max > 0.0) {
+            const double T = 2.0 * gmax * wn_max;
+            F = 50 - (int)std::ceil(std::log2(T));
+            Fa = -std::ilogb(gmax);
+            Fb = F - Fa;
+            if (!std::isfinite(T) || T <= 0.0 || std::abs(Fa) > 1000 || std::abs(Fb) > 1000 || std::abs(F) > 1000)
+                half = false;
+        }
+    }
+    std::vector<Row> rows = half ? half_stencil(h, p->rmax, mst) : full_stencil(h, p->rmax, mst);
 
     GsExtra& X = *v.extra;
     const size_t g_off = ((sizeof(BinDesc) + 15) / 16) * 16;
     const size_t rows_off = g_off + ((size_t)nb * nm * 16 + 15) / 16 * 16;
     const int nblk_red = 256;
     if ((rc = X.params.reserve(rows_off + rows.size() * sizeof(Row) + 16)) || (rc = X.out.reserve(8)) ||
-        (rc = X.queue.reserve(16)) || (rc = X.dbl.reserve(2 * (size_t)n + 2 * nblk_red + 8)))
+        (rc = X.queue.reserve(16)) || (rc = X.dbl.reserve(4 * (size_t)n + 2 * nblk_red + 8)))
         return rc;
     double* G = X.dbl.ptr;
     double* outd = out_dev ? out_dev : X.dbl.ptr + n;
     double* part = X.dbl.ptr + 2 * n;
+    unsigned long long* Gq = (unsigned long long*)(X.dbl.ptr + 2 * n + 2 * nblk_red + 8);   // [n][2]
     hipError_t e = hipMemcpyAsync(X.params.ptr, &B, sizeof(B), hipMemcpyHostToDevice, v.stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(X.params.ptr + g_off, gN2, (size_t)nb * nm * 8, hipMemcpyHostToDevice, v.stream);
@@ -60,6 +84,7 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
     if (e == hipSuccess) e = hipMemsetAsync(X.queue.ptr, 0, 16 * sizeof(int32_t), v.stream);
     if (e == hipSuccess) e = hipMemsetAsync(X.out.ptr, 0, 8 * sizeof(unsigned long long), v.stream);
     if (e == hipSuccess) e = hipMemsetAsync(G, 0, (size_t)std::max<int64_t>(n, 1) * 8, v.stream);
+    if (e == hipSuccess && half) e = hipMemsetAsync(Gq, 0, (size_t)std::max<int64_t>(n, 1) * 16, v.stream);
     if (e != hipSuccess) return hip_fail(e, "adjoint parameter upload");
 
     AdjArgs a{};
@@ -82,12 +107,18 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
     a.n_chunks = v.n_chunks;
     a.inv_wd = inv_wd;
     a.inv_wr = inv_wr;
-    const int occ = adjoint2_occupancy(nb, nm, fp64, rsd);
+    a.Gq = Gq;
+    a.g_scale = std::ldexp(1.0, Fa);
+    a.w_scale = std::ldexp(1.0, Fb);
+    const int occ = half ? adjoint2h_occupancy(nb, nm, rsd) : adjoint2_occupancy(nb, nm, fp64, rsd);
     int nblocks = std::max(1, std::min(v.n_cu * occ, std::max(v.n_chunks, 1)));
     (void)hipEventRecord(v.ev[2], v.stream);
     if (v.n_chunks > 0) {
-        e = launch_adjoint2(a, nb, nm, fp64, rsd, nblocks, v.stream);
+        e = half ? launch_adjoint2h(a, nb, nm, rsd, nblocks, v.stream)
+                 : launch_adjoint2(a, nb, nm, fp64, rsd, nblocks, v.stream);
         if (e != hipSuccess) return hip_fail(e, "adjoint kernel launch");
+        if (half && (e = launch_adj_fx(Gq, v.orig, n, F, G, v.stream)) != hipSuccess)
+            return hip_fail(e, "adjoint fixed-point conversion");
     }
     (void)hipEventRecord(v.ev[3], v.stream);
     e = launch_adj_partials(G, v.w, n, v.n_data, inv_wd, inv_wr, part, nblk_red, v.stream);

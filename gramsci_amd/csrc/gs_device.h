@@ -104,6 +104,28 @@ __device__ __forceinline__ unsigned long long quantise_rn(double x)
     return (unsigned long long)(__double_as_longlong(x + 0x1p52) - 0x4330000000000000ll);
 }
 
+// Rotation of a register across the 64 lanes of a wavefront by one lane (DPP
+// wave_rol:1); every lane must be active.
+// The register itself is passed as the unused "old" operand (a full rotation
+// writes every lane), so the move can be done in place: one v_mov_b32_dpp per
+// 32-bit register and step, with no zeroed temporary and no copy back.
+__device__ __forceinline__ int rot_lane(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xf, 0xf, false); }
+__device__ __forceinline__ float rot_lane(float v) { return __int_as_float(rot_lane(__float_as_int(v))); }
+__device__ __forceinline__ long long rot_lane(long long b)
+{
+    const int lo = rot_lane((int)(b & 0xffffffffll)), hi = rot_lane((int)(b >> 32));
+    return ((long long)hi << 32) | (long long)(uint32_t)lo;
+}
+__device__ __forceinline__ double rot_lane(double v) { return __longlong_as_double(rot_lane(__double_as_longlong(v))); }
+
+// rint(x) as a signed 64-bit integer, for |x| < 2^51: x + 1.5 * 2^52 lies in
+// [2^52, 2^53), where the spacing of doubles is 1, so the addition rounds x to an
+// integer and the bit pattern of the sum is that of 1.5 * 2^52 plus it
+__device__ __forceinline__ long long quantise_rn_signed(double x)
+{
+    return __double_as_longlong(x + 0x1.8p52) - 0x4338000000000000ll;
+}
+
 // largest r with off[r] <= v over off[0..n-1] (off non-decreasing)
 __device__ __forceinline__ int upper_index(const int32_t* off, int n, int v)
 {

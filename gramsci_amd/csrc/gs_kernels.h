@@ -160,10 +160,21 @@ struct AdjArgs {
     int32_t nrows;
     int32_t n_chunks;
     double inv_wd, inv_wr;
+    // half-stencil FP32 kernel: per sorted point a 128-bit fixed-point sum [n][2] (lo, hi; zeroed before
+    // launch) of G * 2^F, with the dL/dN tables scaled by g_scale and the normalised weights by w_scale
+    // (g_scale * w_scale = 2^F)
+    unsigned long long* Gq;
+    double g_scale, w_scale;
 };
 size_t adjoint2_lds_bytes(int nb, int nm, bool fp64);
 hipError_t launch_adjoint2(const AdjArgs& a, int nb, int nm, bool fp64, bool rsd, int nblocks, hipStream_t st);
 int adjoint2_occupancy(int nb, int nm, bool fp64, bool rsd);
+// half-stencil FP32 adjoint (rows: half stencil): every unordered pair once, its term added to both ends
+size_t adjoint2h_lds_bytes(int nb, int nm);
+hipError_t launch_adjoint2h(const AdjArgs& a, int nb, int nm, bool rsd, int nblocks, hipStream_t st);
+int adjoint2h_occupancy(int nb, int nm, bool rsd);
+// G[orig[k]] = Gq[k] * 2^-F (the 128-bit sums as doubles, original order)
+hipError_t launch_adj_fx(const unsigned long long* Gq, const int32_t* orig, int64_t n, int F, double* G, hipStream_t st);
 hipError_t launch_adj_partials(const double* G, const double* w, int64_t n, int64_t n_data, double inv_wd,
                                double inv_wr, double* part, int nblk, hipStream_t st);
 hipError_t launch_adj_finish(const double* G, int64_t n, int64_t n_data, double s_d, double s_r, double inv_wd,

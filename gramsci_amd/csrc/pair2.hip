@@ -52,20 +52,6 @@ struct Stats {
     unsigned long long cand, rechk, over;
 };
 
-// Rotation of a register across the 64 lanes of a wavefront by one lane (DPP
-// wave_rol:1); every lane must be active.
-// The register itself is passed as the unused "old" operand (a full rotation
-// writes every lane), so the move can be done in place: one v_mov_b32_dpp per
-// register and step, with no zeroed temporary and no copy back.
-__device__ __forceinline__ int rot_lane(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x134, 0xf, 0xf, false); }
-__device__ __forceinline__ float rot_lane(float v) { return __int_as_float(rot_lane(__float_as_int(v))); }
-__device__ __forceinline__ double rot_lane(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = rot_lane((int)(b & 0xffffffffll)), hi = rot_lane((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
-}
-
 // One pair's histogram update.  base = class offset + b*nm (flat bin index
 // without the mu bin); counts live at ((base + m) << rsh) + rep.  mirrored:
 // mr == nm-1-m (always so for the FP32 mu bins; tested after an FP64 recheck).

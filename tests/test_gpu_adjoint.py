@@ -109,3 +109,41 @@ def test_autograd_theta_vs_finite_differences():
             fd[k] = (loss_of(tp).item() - loss_of(tm).item()) / (2 * h)
     print("dL/dtheta", grad, "fd", fd, flush=True)
     assert np.all(np.abs(grad - fd) <= 1e-6 * np.max(np.abs(fd)) + 1e-9 * np.abs(fd).max())
+
+
+def test_adjoint_fp32_reproducible_and_matches_fp64_at_scale(eng):
+    # 120k points at the density of the differentiable configuration, with buffer points: the FP32
+    # half-stencil kernel (each pair once, terms to both ends, exact fixed-point sums) gives the same
+    # bits on every call and agrees with the full-stencil FP64 kernel
+    side = (120_000 / 2e-3) ** (1.0 / 3.0)
+    d = C.clustered(30_000, side, 61, members=5, sigma=2.0)
+    r = C.uniform(90_000, side, 62)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(30_000, 0.3, 63), C.lognormal_weights(90_000, 0.05, 64)])
+    buf = (np.random.default_rng(65).random(len(xyz)) < 0.1).astype(np.int32)
+    eng.set_catalogue(xyz, w, len(d), buf)
+    rng = np.random.default_rng(66)
+    g2 = rng.normal(0, 1, (20, 10))
+    g3 = rng.normal(0, 1, (20, 10)) * 1e-3
+    args = (0.0, 120.0, 20, 10, False)
+    a = eng.adjoint2(g2, g3, *args, "fp32")
+    b = eng.adjoint2(g2, g3, *args, "fp32")
+    c = eng.adjoint2(g2, g3, *args, "fp64")
+    assert np.array_equal(a, b)
+    err = np.max(np.abs(a - c)) / np.max(np.abs(c))
+    print(f"fp32 half-stencil vs fp64: {err:.3g}", flush=True)
+    assert err < 1e-10
+
+
+def test_adjoint_nonfinite_gradient_propagates(eng):
+    # a non-finite dL/dN entry takes the FP64-summing kernel in both precisions: NaN propagates as in
+    # the forward arithmetic instead of being rounded into the fixed-point sums
+    d, r, t, theta, side = _subset_c5(seed=2)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([np.exp(t @ theta), np.ones(len(r))])
+    eng.set_catalogue(xyz, w, len(d))
+    g2 = np.ones((20, 10))
+    g2[3, 4] = np.nan
+    for prec in ("fp32", "fp64"):
+        g = eng.adjoint2(g2, np.zeros((20, 10)), 0.0, 120.0, 20, 10, False, prec)
+        assert np.isnan(g).any(), prec

@@ -108,7 +108,10 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nzg - 1);
 
         const unsigned long long cand0 = cand, over0 = over;   // restored if the apex is queued
-        // ---- 1. gather
+        // ---- 1. gather: the candidates of all stencil rows are numbered consecutively and
+        // thread t takes numbers t, t + 256, ... (every thread busy even where rows are short)
+        int flat = 0;           // candidates in the rows before this one (work-group uniform)
+        int next = tid;         // this thread's next candidate number
         for (int r = 0; r < a.nrows; ++r) {
             const Row rw = a.rows[r];
             const int by = cy + rw.dy, bz = cz + rw.dz;
@@ -117,7 +120,11 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             if (bx0 > bx1) continue;
             const int rb = (bz * nyg + by) * nxg;
             const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
-            for (int j = js + tid; j < je; j += kWG) {
+            const int row_end = flat + (je - js);
+            const int j0 = js - flat;
+            flat = row_end;
+            for (; next < row_end; next += kWG) {
+                const int j = j0 + next;
                 ++cand;
                 const double4 Pj = a.pw[j];
                 const int b = bin_of(exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z), t64, nb);

@@ -233,6 +233,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
             for (int k = 0; k < 3; ++k) vol *= std::max(B.bbox[6 * r + 3 + k] - B.bbox[6 * r + k], p.rmax);
             kmean = std::max(kmean, (double)(B.off[r + 1] - B.off[r]) * shell / vol);
         }
+        ta1.wg = triplet_wg(kmean);
         int kmax1 = std::min(ta.kmax, std::max(128, (int)std::ceil(2.0 * kmean / 64.0) * 64));
         const char* ev = getenv("GS_T3_KMAX");      // tuning override (unset or empty: automatic)
         if (ev && *ev) kmax1 = std::max(64, std::min(ta.kmax, atoi(ev)));
@@ -243,6 +244,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
             ta2.items = B.ovf.ptr + 1;
             ta2.n_items_dev = B.ovf.ptr;
             ta2.queue = B.queue.ptr + 16;
+            ta2.wg = kWG;           // the apexes with the most neighbours, one per fetch
+            ta2.batch = 1;
             nblocks2 = v.n_cu * triplet3_occupancy(ta2, nb, fp64, weighted);
         }
     }
@@ -250,6 +253,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
                               : std::max(1, pair2_occupancy(nb, nm, 1, fp64, rsd, weighted, p.log_bins != 0, true));
     const int nblocks = kind == 3 ? std::max(1, (int)std::min<int64_t>((int64_t)v.n_cu * occ, std::max<int64_t>(n, 1)))
                                   : v.n_cu * occ;
+    if (kind == 3) ta1.batch = triplet_batch(n, nblocks);
 
     // capture: clear, count sort into cells, scans, scatter, row chunks, pair or triplet kernel
     e = hipStreamBeginCapture(v.stream, hipStreamCaptureModeThreadLocal);

@@ -144,6 +144,26 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Next batch of up to bsz work items from the 8 queues of next_item below: returns
+// the first item and sets `end` (exclusive), or returns -1 at the end.  One atomic
+// per batch: with millions of short work items (apexes with few neighbours) one
+// atomic per item would queue on the counters' cache line.
+__device__ __forceinline__ int next_items(int32_t* queue, int n, int xcd, int& tried, int bsz, int& end)
+{
+    while (tried < 8) {
+        const int sidx = (xcd + tried) & 7;
+        const int lo = (int)(((long long)n * sidx) >> 3);
+        const int hi = (int)(((long long)n * (sidx + 1)) >> 3);
+        const int k = atomicAdd(&queue[sidx], bsz);
+        if (lo + k < hi) {
+            end = min(lo + k + bsz, hi);
+            return lo + k;
+        }
+        ++tried;
+    }
+    return -1;
+}
+
 // Next work item from 8 queues, one per XCD segment of [0, n): own segment
 // first, then the others.  `tried` is per-caller state.  Returns -1 at the end.
 __device__ __forceinline__ int next_item(int32_t* queue, int n, int xcd, int& tried)

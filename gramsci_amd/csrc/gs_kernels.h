@@ -103,6 +103,8 @@ struct TripArgs {
     const int32_t* items;
     const int32_t* n_items_dev;     // if set, the number of work items is read here (captured second pass)
     int32_t log_bins;               // selects the kernel variant (log or linear bin estimate)
+    int32_t wg;                     // threads per work-group (one apex each): 64, 128 or 256 (0: 256)
+    int32_t batch;                  // apexes taken per queue fetch (0: 1)
 };
 size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);
@@ -136,6 +138,7 @@ struct QuadArgs {
     int32_t* ovf;
     int32_t* ovf_n;
     const int32_t* items;
+    int32_t batch;              // apexes taken per queue fetch (0: 1)
 };
 size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra);
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);

@@ -110,6 +110,25 @@ std::vector<uint8_t> normal_table(int nm)
 // LDS plan of the triplet kernel for a.ncfg/a.nm: histogram replicas and
 // neighbour capacity within 64 KiB; a histogram too large for LDS goes to
 // global atomics instead.
+int triplet_wg(double kmean)
+{
+    const char* ev = getenv("GS_T3_WG");        // tuning override (unset or empty: automatic)
+    if (ev && *ev) {
+        const int w = atoi(ev);
+        if (w == 64 || w == 128 || w == 256) return w;
+    }
+    // 256 threads per apex measured fastest also for short neighbour lists (64 batched regions,
+    // mean ~100 neighbours: 108 ms per pass with 256 threads, 153 ms with 64), once work items are
+    // fetched in batches; smaller work-groups stay available for other shapes
+    (void)kmean;
+    return 256;
+}
+
+int triplet_batch(int64_t n, int nblocks)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(16, n / (128 * (int64_t)std::max(nblocks, 1))));
+}
+
 int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted)
 {
     const size_t nh = (size_t)a.ncfg * a.nm * 4;
@@ -245,9 +264,11 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
         if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
         if (e != hipSuccess) return hip_fail(e, "3PCF neighbour sample");
         std::vector<int32_t> ks;
+        double ksum = 0.0;
         for (int32_t k : kc)
-            if (k >= 0) ks.push_back(k);
+            if (k >= 0) { ks.push_back(k); ksum += k; }
         if (!ks.empty()) {
+            a.wg = triplet_wg(ksum / (double)ks.size());
             const size_t q = std::min(ks.size() - 1, (size_t)std::ceil(0.995 * (ks.size() - 1)));
             std::nth_element(ks.begin(), ks.begin() + q, ks.end());
             const int k995 = ks[q];
@@ -270,7 +291,9 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
             if (e != hipSuccess) return hip_fail(e, "3PCF overflow counter");
         }
         const int occ = triplet3_occupancy(a1, nb, fp64, weighted);
-        e = launch_triplet3(a1, nb, fp64, weighted, std::max(1, std::min(v.n_cu * occ, n)), v.stream);
+        const int nblk1 = std::max(1, std::min(v.n_cu * occ, n));
+        a1.batch = triplet_batch(n, nblk1);
+        e = launch_triplet3(a1, nb, fp64, weighted, nblk1, v.stream);
         if (e != hipSuccess) return hip_fail(e, "3PCF kernel launch");
         if (two_pass) {
             int32_t nov = 0;
@@ -281,6 +304,8 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
                 TripArgs a2 = a;
                 a2.items = X.i32.ptr + 1;
                 a2.n_items = nov;
+                a2.wg = kWG;            // the apexes with the most neighbours, one per fetch
+                a2.batch = 1;
                 e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);   // work counters only
                 const int occ2 = triplet3_occupancy(a2, nb, fp64, weighted);
                 if (e == hipSuccess)
@@ -461,7 +486,9 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
             if (e != hipSuccess) return hip_fail(e, "4PCF overflow counter");
         }
         const int occ = quad4_occupancy(nb, ncell, a1.kmax, lds_hist, tetra, weighted);
-        e = launch_quad4(a1, nb, weighted, std::max(1, std::min(v.n_cu * occ, n)), v.stream);
+        const int nblk1 = std::max(1, std::min(v.n_cu * occ, n));
+        a1.batch = triplet_batch(n, nblk1);
+        e = launch_quad4(a1, nb, weighted, nblk1, v.stream);
         if (e != hipSuccess) return hip_fail(e, "4PCF kernel launch");
         if (two_pass) {
             int32_t nov = 0;
@@ -472,6 +499,7 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
                 QuadArgs a2 = a;
                 a2.items = X.i32.ptr + 1;
                 a2.n_items = nov;
+                a2.batch = 1;
                 e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);   // work counters only
                 const int occ2 = quad4_occupancy(nb, ncell, kmax, lds_hist, tetra, weighted);
                 if (e == hipSuccess)

@@ -86,10 +86,13 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     int tried = 0;
     const int nb2 = nb * nb;
 
+    const int bsz = max(a.batch, 1);
+    int b_next = 0, b_end = 0;      // thread 0: the work-group's current batch of apexes
     while (true) {
         if (tid == 0) {
-            const int it = next_item(a.queue, a.n_items, xcd, tried);
-            s_item = it;
+            if (b_next >= b_end) b_next = next_items(a.queue, a.n_items, xcd, tried, bsz, b_end);
+            s_item = b_next;        // -1 at the end
+            if (b_next >= 0) ++b_next;
             s_k = 0;
             s_over = 0;
         }

@@ -73,6 +73,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const bool lh = !GEN || a.lds_hist != 0;
     constexpr bool equi = (MODE & kT3Equi) != 0;
     const int tid = threadIdx.x;
+    const int nt = blockDim.x;              // threads per work-group: 256, or 64 for short neighbour lists
     const int lane = tid & 63;
     const int rep = lane & (reps - 1);
     const int rsh = 31 - __clz(reps);       // reps is a power of two
@@ -105,16 +106,16 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     __shared__ int s_item, s_k, s_over;
 
     if (lh) {
-        for (int k = tid; k < nh * reps; k += kWG) hc[k] = 0;
+        for (int k = tid; k < nh * reps; k += nt) hc[k] = 0;
         if (W)
-            for (int k = tid; k < 2 * nh; k += kWG) hw[k] = 0;
+            for (int k = tid; k < 2 * nh; k += nt) hw[k] = 0;
     }
-    for (int k = tid; k < nb; k += kWG) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
-    for (int k = tid; k < nb + 2; k += kWG) t64[k] = B.t64[k];
+    for (int k = tid; k < nb; k += nt) okb[k] = make_float2(B.ok_lo[k], B.ok_hi[k]);
+    for (int k = tid; k < nb + 2; k += nt) t64[k] = B.t64[k];
     if (!equi)
-        for (int k = tid; k < nb * nb * nb; k += kWG) tab[k] = a.cfg_tab[k];
+        for (int k = tid; k < nb * nb * nb; k += nt) tab[k] = a.cfg_tab[k];
     if (nm > 1)
-        for (int k = tid; k < nm * nm; k += kWG) ntab[k] = a.ntab[k];
+        for (int k = tid; k < nm * nm; k += nt) ntab[k] = a.ntab[k];
     __syncthreads();
     const double ms = B.mu_scale;
 
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     auto drain = [&](int r) {
         unsigned long long* oc = a.out_cnt + (size_t)r * nh;
         unsigned long long* ow = a.out_w + (size_t)r * 2 * nh;
-        for (int k = tid; k < nh; k += kWG) {
+        for (int k = tid; k < nh; k += nt) {
             unsigned long long sum = 0;
             for (int q = 0; q < reps; ++q) { sum += hc[k * reps + q]; hc[k * reps + q] = 0; }
             if (sum) atomicAdd(&oc[k], sum);
@@ -153,10 +154,13 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     int tried = 0;
 
     const int n_items = a.n_items_dev ? *a.n_items_dev : a.n_items;
+    const int bsz = max(a.batch, 1);
+    int b_next = 0, b_end = 0;      // thread 0: the work-group's current batch of apexes
     while (true) {
         if (tid == 0) {
-            const int it = next_item(a.queue, n_items, xcd, tried);
-            s_item = it;
+            if (b_next >= b_end) b_next = next_items(a.queue, n_items, xcd, tried, bsz, b_end);
+            s_item = b_next;        // -1 at the end
+            if (b_next >= 0) ++b_next;
             s_k = 0;
             s_over = 0;
         }
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 
         const unsigned long long cand0 = cand, over0 = over;   // restored if the apex is queued
         // ---- 1. gather the kept neighbours with their exact bin.  The candidates of all
-        // stencil rows are numbered consecutively and thread t takes numbers t, t + 256, ...,
+        // stencil rows are numbered consecutively and thread t takes numbers t, t + nt, ...,
         // so every thread is busy even where a row holds fewer points than the work-group
         // has threads (coarse cells at low density).
         int flat = 0;           // candidates in the rows before this one (work-group uniform)
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             const int row_end = flat + (je - js);
             const int j0 = js - flat;
             flat = row_end;
-            for (; next < row_end; next += kWG) {
+            for (; next < row_end; next += nt) {
                 const int j = j0 + next;
                 ++cand;
                 const double4 Pj = a.pw[j];
@@ -267,12 +271,12 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         // ---- 2. neighbour pairs
         // The triangle u < v is folded into rows of K: pair index pi = r*K + c is
         // (u, v) = (r, r+1+c) for c < K-1-r, else (K-2-r, c) (row K-2-r's pairs).
-        // A lane's index advances by 256 per step, i.e. c by 256 mod K with at most
+        // A lane's index advances by the work-group size per step, i.e. c by that mod K with at most
         // one carry into r: no loop, and neighbouring lanes read neighbouring entries.
         const int Kc = max(K, 1);
-        const int dq = kWG / Kc, dr = kWG - dq * Kc;
+        const int dq = nt / Kc, dr = nt - dq * Kc;
         int r = tid / Kc, c = tid - r * Kc;
-        for (int pi = tid; pi < P; pi += kWG) {
+        for (int pi = tid; pi < P; pi += nt) {
             const bool first = c < K - 1 - r;
             const int u = first ? r : K - 2 - r;
             const int v = first ? r + 1 + c : c;
@@ -416,7 +420,7 @@ static hipError_t launch_t3(T3Kernel kern, const TripArgs& a, size_t lds, int nb
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(kWG), lds, st, a);
+    hipLaunchKernelGGL(kern, dim3(nblocks), dim3(a.wg > 0 ? a.wg : kWG), lds, st, a);
     return hipGetLastError();
 }
 
@@ -478,7 +482,7 @@ int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted)
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return 1;
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kWG, lds) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, a.wg > 0 ? a.wg : kWG, lds) != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 

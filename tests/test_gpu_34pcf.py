@@ -155,6 +155,28 @@ def test_3pcf_two_pass_overflow(eng, monkeypatch, nmu):
         assert o["stats"]["n_overflow"] == base["stats"]["n_overflow"]
 
 
+@pytest.mark.parametrize("wg", ["64", "128", "256"])
+@pytest.mark.parametrize("nmu,weighted", [(1, True), (3, False)])
+def test_3pcf_work_group_sizes(eng, monkeypatch, wg, nmu, weighted):
+    # one apex per work-group of 64, 128 or 256 threads (256 unless GS_T3_WG says otherwise):
+    # counts equal the oracle's, sums and statistics equal across sizes
+    rng = np.random.default_rng(13)
+    d = C.clustered(3000, 90.0, 85)
+    r = C.uniform(6000, 90.0, 86)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(3000, 0.2, 87), np.ones(6000)]) if weighted else None
+    buf = (rng.random(len(xyz)) < 0.1).astype(np.int32)
+    p = O.Params(2.0, 12.0, 4, nmu, False)
+    monkeypatch.setenv("GS_T3_WG", "256")
+    eng.set_catalogue(xyz, w, 3000, buf)
+    base = eng.pcf3(p.rmin, p.rmax, p.nbins, nmu=nmu, precision="fp32")
+    monkeypatch.setenv("GS_T3_WG", wg)
+    ref, outs = check3(eng, xyz, w, 3000, p, buf=buf)
+    for o in outs:
+        assert np.array_equal(o["cnt"], base["cnt"]) and np.array_equal(o["wsum"], base["wsum"])
+        assert o["stats"]["n_candidates"] == base["stats"]["n_candidates"]
+
+
 def test_3pcf_config3_density_slab(eng):
     # a 150^3 slab at the density of the 3PCF configuration (600k points in 500^3)
     d = C.clustered(5400, 150.0, 81)

@@ -1,6 +1,6 @@
 """CPU check of the folded neighbour-pair schedule of the triplet kernel
-(gramsci_amd/csrc/triplet3.hip, step 2): 256 threads, thread t starting at
-pair index t and advancing by 256; pair index pi = r*K + c maps to
+(gramsci_amd/csrc/triplet3.hip, step 2): n threads (256, or 64 / 128 for short
+neighbour lists), thread t starting at pair index t and advancing by n; pair index pi = r*K + c maps to
 (u, v) = (r, r+1+c) if c < K-1-r, else (K-2-r, c).  Every unordered pair
 u < v of K neighbours must be visited exactly once (each apex pair is one
 triangle candidate of the reference's k1 < k2 loops, gramsci.F90:479-489)."""
@@ -29,9 +29,12 @@ def schedule(K, nthr=NTHR):
             r += dq + (1 if carry else 0)
 
 
-@pytest.mark.parametrize("K", list(range(0, 40)) + [127, 128, 129, 255, 256, 257, 511, 512, 513, 768, 1000, 4096])
-def test_every_pair_once(K):
-    got = list(schedule(K))
+@pytest.mark.parametrize("nthr", [64, 128, 256])
+@pytest.mark.parametrize("K", list(range(0, 40)) + [63, 64, 65, 127, 128, 129, 255, 256, 257, 511, 512, 513, 768, 1000,
+                                                    4096])
+def test_every_pair_once(K, nthr):
+    # the kernel runs 256 threads per apex, or 64 / 128 for short neighbour lists
+    got = list(schedule(K, nthr))
     assert all(0 <= u < v < K for u, v in got)
     assert len(got) == K * (K - 1) // 2
     assert len(set(got)) == len(got)

@@ -184,7 +184,9 @@ int gs_decompose(gs_ctx* ctx, const float* xyz, const int32_t* region, int64_t n
  * dL/dw_k for the RAW weights w_k (the normalisation of gramsci.F90:1390-1391
  * is differentiated through).  The pairs, bins and buffer rule are those of
  * gs_count_2pcf (query_graph_2pcf, gramsci.F90:352-380); the reference has no
- * derivative.  dl_dw[n] in host memory, or d_dl_dw[n] in device memory. */
+ * derivative.  dl_dw[n] in host memory, or d_dl_dw[n] in device memory.
+ * GS_FP32_GUARD: per-point sums in exact fixed point (bit-identical across
+ * calls); a non-finite dL/dN entry switches to FP64 sums, so NaN propagates. */
 int gs_adjoint_2pcf(gs_ctx* ctx, const gs_params* p, const double* dl_dn2, const double* dl_dn3,
                     double* dl_dw, gs_stats* st);
 int gs_adjoint_2pcf_device(gs_ctx* ctx, const gs_params* p, const double* dl_dn2, const double* dl_dn3,

@@ -66,12 +66,13 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
     const size_t rows_off = g_off + ((size_t)nb * nm * 16 + 15) / 16 * 16;
     const int nblk_red = 256;
     if ((rc = X.params.reserve(rows_off + rows.size() * sizeof(Row) + 16)) || (rc = X.out.reserve(8)) ||
-        (rc = X.queue.reserve(16)) || (rc = X.dbl.reserve(4 * (size_t)n + 2 * nblk_red + 8)))
+        (rc = X.queue.reserve(16)) ||
+        (rc = X.dbl.reserve(2 * (size_t)n + 2 * nblk_red + 8 + 2 * (size_t)std::max<int64_t>(n, 1))))
         return rc;
     double* G = X.dbl.ptr;
     double* outd = out_dev ? out_dev : X.dbl.ptr + n;
     double* part = X.dbl.ptr + 2 * n;
-    unsigned long long* Gq = (unsigned long long*)(X.dbl.ptr + 2 * n + 2 * nblk_red + 8);   // [n][2]
+    unsigned long long* Gq = (unsigned long long*)(X.dbl.ptr + 2 * n + 2 * nblk_red + 8);   // [max(n, 1)][2]
     hipError_t e = hipMemcpyAsync(X.params.ptr, &B, sizeof(B), hipMemcpyHostToDevice, v.stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(X.params.ptr + g_off, gN2, (size_t)nb * nm * 8, hipMemcpyHostToDevice, v.stream);

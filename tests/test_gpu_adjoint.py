@@ -147,3 +147,21 @@ def test_adjoint_nonfinite_gradient_propagates(eng):
     for prec in ("fp32", "fp64"):
         g = eng.adjoint2(g2, np.zeros((20, 10)), 0.0, 120.0, 20, 10, False, prec)
         assert np.isnan(g).any(), prec
+
+
+def test_adjoint_tiny_and_empty_catalogues(eng):
+    # 0 to 3 points, with and without randoms, in both precisions: the oracle's gradient (an empty
+    # catalogue gives an empty one)
+    pts = np.array([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [2.0, 0.5, 0.0]])
+    g2 = np.arange(8.0).reshape(4, 2)
+    g3 = np.ones((4, 2))
+    p = O.Params(0.5, 3.0, 4, 2, False)
+    for n, nd in ((0, 0), (1, 1), (2, 1), (3, 2), (3, 3)):
+        xyz = pts[:n]
+        w = np.linspace(1.0, 2.0, n)
+        eng.set_catalogue(xyz, w, nd)
+        ref = O.adjoint2(xyz, w, nd, p, g2, g3) if n else np.zeros(0)
+        for prec in ("fp32", "fp64"):
+            g = eng.adjoint2(g2, g3, p.rmin, p.rmax, p.nbins, p.nmu, False, prec)
+            assert g.shape == (n,), (n, prec)
+            assert np.all(np.abs(g - ref) <= 1e-12 * max(np.abs(ref).max() if n else 0.0, 1.0)), (n, nd, prec, g, ref)

@@ -101,6 +101,28 @@ def test_config1_full_size(eng):
     assert int(ref["cnt"].sum()) > 1e7
 
 
+def test_config2_full_size_properties(eng):
+    # the (r, mu) configuration at its real size (1M Neyman-Scott galaxies with lognormal weights +
+    # 3M uniform randoms in 1000^3, 1 < r < 150, 30 log bins, nmu = 20), where the oracle would take
+    # days: the FP32 + guard path gives the FP64 path's counts and fixed-point sums bit for bit, at two
+    # cell sizes, and every class's per-r-bin total is even (each pair is counted in both orientations
+    # with the same exact distance)
+    d = C.neyman_scott(500_000, 25_000, 20, 2.0, 1000.0, 201)
+    r = C.uniform(3_000_000, 1000.0, 202)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(len(d), 0.2, 203), np.ones(len(r))])
+    eng.set_catalogue(xyz, w, len(d))
+    args = (1.0, 150.0, 30, 20, True)
+    f32 = eng.pcf2(*args, "fp32")
+    f64 = eng.pcf2(*args, "fp64")
+    f32m = eng.pcf2(*args, "fp32", cell_div=5)
+    print("C2 full size: binned", int(f32["cnt"].sum()), "rechecks", f32["stats"]["n_recheck"], flush=True)
+    assert np.array_equal(f32["cnt"], f64["cnt"]) and np.array_equal(f32["wsum"], f64["wsum"])
+    assert np.array_equal(f32m["cnt"], f32["cnt"]) and np.array_equal(f32m["wsum"], f32["wsum"])
+    assert np.all(f32["cnt"].sum(axis=2) % 2 == 0)
+    assert int(f32["cnt"].sum()) > 1e11
+
+
 def test_rmu_log_weighted_clustered(eng):
     d = C.clustered(30000, 300.0, 7)
     r = C.uniform(60000, 300.0, 8)

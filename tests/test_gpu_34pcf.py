@@ -177,6 +177,25 @@ def test_3pcf_work_group_sizes(eng, monkeypatch, wg, nmu, weighted):
         assert o["stats"]["n_candidates"] == base["stats"]["n_candidates"]
 
 
+def test_3pcf_config3_full_size_properties(eng):
+    # the 3PCF configuration at its real size (200k Neyman-Scott galaxies + 400k uniform randoms in
+    # 500^3, 10 < r < 30, 5 bins; ~7e10 apex-pair candidates): FP32 + guard counts equal FP64's at two
+    # cell sizes, and every (configuration, class) count is a multiple of 3 (each triangle is counted
+    # once from each of its three apexes)
+    d = C.neyman_scott(100000, 5000, 20, 2.0, 500.0, 301)
+    r = C.uniform(400000, 500.0, 302)
+    xyz = np.concatenate([d, r])
+    eng.set_catalogue(xyz, None, len(d))
+    a = eng.pcf3(10.0, 30.0, 5, precision="fp32")
+    b = eng.pcf3(10.0, 30.0, 5, precision="fp64")
+    c = eng.pcf3(10.0, 30.0, 5, precision="fp32", cell_div=2)
+    print("C3 full size: units", a["stats"]["n_units"], "binned", a["stats"]["n_binned"], flush=True)
+    assert np.array_equal(a["cnt"], b["cnt"]) and np.array_equal(a["cnt"], c["cnt"])
+    assert a["stats"]["n_units"] == b["stats"]["n_units"] == c["stats"]["n_units"]
+    assert np.all(a["cnt"] % 3 == 0)
+    assert a["stats"]["n_units"] > 5e10
+
+
 def test_3pcf_config3_density_slab(eng):
     # a 150^3 slab at the density of the 3PCF configuration (600k points in 500^3)
     d = C.clustered(5400, 150.0, 81)

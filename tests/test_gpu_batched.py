@@ -142,3 +142,23 @@ def test_batch_survives_single_catalogue_uploads(eng, batch):
         sl = slice(off[r], off[r + 1])
         ref = O.pcf3(xyz[sl], None, int(ndata[r]), O.Params(2.0, 8.0, 4), buf=buf[sl])
         assert np.array_equal(a3["cnt"][r][:, 0], ref["cnt"]), r
+
+
+def test_batched_full_size_equals_separate_launches(eng):
+    # the 64-region configuration at its real size (28M points with their rmax buffer shells):
+    # one captured launch over all regions gives, region by region, the counts of one plain launch
+    # per region, for the 2PCF (5 < r < 100, 20 bins, nmu = 10) and the 3PCF (5 < r < 25, 4 bins)
+    xyz, buf, off, ndata = C.regions(64, 401, rmax=100.0)
+    eng.set_catalogue_batched(xyz, None, buf, off, ndata)
+    p2 = (5.0, 100.0, 20, 10, False, "fp32")
+    p3 = (5.0, 25.0, 4, 1, False, "fp32")
+    a2 = eng.pcf2_batched(*p2)
+    a3 = eng.pcf3_batched(*p3)
+    m2, m3 = a2["stats"]["cell_div"], a3["stats"]["cell_div"]
+    print("64 regions:", len(xyz), "points, 2PCF binned", a2["stats"]["n_binned"], "m", m2,
+          "3PCF binned", a3["stats"]["n_binned"], "m", m3, flush=True)
+    for k in range(len(ndata)):
+        sl = slice(off[k], off[k + 1])
+        eng.set_catalogue(xyz[sl], None, int(ndata[k]), buf[sl])
+        assert np.array_equal(eng.pcf2(*p2[:5], p2[5], cell_div=m2)["cnt"], a2["cnt"][k]), k
+        assert np.array_equal(eng.pcf3(*p3[:3], nmu=1, precision="fp32", cell_div=m3)["cnt"], a3["cnt"][k][:, 0]), k

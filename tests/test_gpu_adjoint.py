@@ -165,3 +165,32 @@ def test_adjoint_tiny_and_empty_catalogues(eng):
             g = eng.adjoint2(g2, g3, p.rmin, p.rmax, p.nbins, p.nmu, False, prec)
             assert g.shape == (n,), (n, prec)
             assert np.all(np.abs(g - ref) <= 1e-12 * max(np.abs(ref).max() if n else 0.0, 1.0)), (n, nd, prec, g, ref)
+
+
+def test_adjoint_full_size_properties(eng):
+    # the differentiable configuration at its real size (500k Neyman-Scott galaxies with template
+    # weights exp(theta . t) + 1.5M randoms in 1000^3, 0 < r < 120, 20 bins, nmu = 10): the FP32
+    # half-stencil gradient is bit-identical from call to call, agrees with the full-stencil FP64
+    # kernel, and satisfies the scale invariance sum over each catalogue of w dL/dw = 0
+    d = C.neyman_scott(250000, 12500, 20, 2.0, 1000.0, 501)
+    r = C.uniform(1500000, 1000.0, 502)
+    fields = C.grf_templates(8, 128, 1000.0, 50.0, 503)
+    t = C.sample_trilinear(fields, d, 1000.0)
+    theta = np.random.default_rng(504).normal(0, 0.05, 8)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([np.exp(t @ theta), np.ones(len(r))])
+    eng.set_catalogue(xyz, w, len(d))
+    rng = np.random.default_rng(505)
+    g2 = rng.normal(0, 1, (20, 10))
+    g3 = rng.normal(0, 1, (20, 10)) * 1e-2
+    args = (0.0, 120.0, 20, 10, False)
+    a = eng.adjoint2(g2, g3, *args, "fp32")
+    b = eng.adjoint2(g2, g3, *args, "fp32")
+    c = eng.adjoint2(g2, g3, *args, "fp64")
+    err = np.max(np.abs(a - c)) / np.max(np.abs(c))
+    print(f"2M-point adjoint: fp32 vs fp64 {err:.3g}", flush=True)
+    assert np.array_equal(a, b)
+    assert err < 1e-10
+    s = np.abs(w * a).sum()
+    assert abs(np.dot(w[:len(d)], a[:len(d)])) < 1e-12 * s
+    assert abs(np.dot(w[len(d):], a[len(d):])) < 1e-12 * s

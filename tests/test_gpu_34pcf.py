@@ -196,6 +196,28 @@ def test_3pcf_config3_full_size_properties(eng):
     assert a["stats"]["n_units"] > 5e10
 
 
+def test_4pcf_config3_full_size_properties(eng, monkeypatch):
+    # the 4PCF configuration at its real size (a seeded 50k-galaxy + 100k-random subset of the 3PCF
+    # catalogue, 10 < r < 20, 3 bins): counts and statistics equal at two cell sizes and with a
+    # forced 64-entry first-pass neighbour list (most apexes then go to the second pass)
+    d = C.neyman_scott(100000, 5000, 20, 2.0, 500.0, 301)
+    r = C.uniform(400000, 500.0, 302)
+    rng = np.random.default_rng(303)
+    d = d[np.sort(rng.choice(len(d), 50000, replace=False))]
+    r = r[np.sort(rng.choice(len(r), 100000, replace=False))]
+    xyz = np.concatenate([d, r])
+    eng.set_catalogue(xyz, None, len(d))
+    a = eng.pcf4(10.0, 20.0, 3)
+    b = eng.pcf4(10.0, 20.0, 3, cell_div=2)
+    monkeypatch.setenv("GS_T4_KMAX", "64")
+    c = eng.pcf4(10.0, 20.0, 3)
+    print("4PCF full size: units", a["stats"]["n_units"], "binned", a["stats"]["n_binned"], flush=True)
+    for o in (b, c):
+        assert np.array_equal(o["cnt"], a["cnt"])
+        assert o["stats"]["n_units"] == a["stats"]["n_units"] and o["stats"]["n_binned"] == a["stats"]["n_binned"]
+    assert a["stats"]["n_binned"] > 0
+
+
 def test_3pcf_config3_density_slab(eng):
     # a 150^3 slab at the density of the 3PCF configuration (600k points in 500^3)
     d = C.clustered(5400, 150.0, 81)

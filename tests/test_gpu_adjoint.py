@@ -194,3 +194,24 @@ def test_adjoint_full_size_properties(eng):
     s = np.abs(w * a).sum()
     assert abs(np.dot(w[:len(d)], a[:len(d)])) < 1e-12 * s
     assert abs(np.dot(w[len(d):], a[len(d):])) < 1e-12 * s
+
+
+@pytest.mark.parametrize("log,buffered", [(False, False), (True, True)])
+def test_adjoint_isotropic(eng, log, buffered):
+    # nmu = 1 (no mu binning): the isotropic variants of both adjoint kernels against the oracle
+    rng = np.random.default_rng(7)
+    d = C.clustered(2500, 90.0, 71, members=5)
+    r = C.uniform(5000, 90.0, 72)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(2500, 0.3, 73), C.lognormal_weights(5000, 0.1, 74)])
+    buf = (rng.random(len(xyz)) < 0.2).astype(np.int32) if buffered else None
+    p = O.Params(1.0 if log else 0.0, 20.0, 8, 1, log)
+    ref = O.pcf2(xyz, w, 2500, p, buf=buf)
+    _, g2, g3 = _loss_grads(ref["n2"], ref["n3"], np.full((8, 1), 0.05))
+    g_ref = O.adjoint2(xyz, w, 2500, p, g2, g3, buf=buf)
+    eng.set_catalogue(xyz, w, 2500, buf)
+    for prec in ("fp32", "fp64"):
+        g = eng.adjoint2(g2, g3, p.rmin, p.rmax, p.nbins, 1, log, prec)
+        err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
+        print(f"isotropic adjoint {prec} log={log} buffered={buffered}: {err:.3g}", flush=True)
+        assert err < 1e-10

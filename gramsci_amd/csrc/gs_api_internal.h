@@ -121,6 +121,7 @@ int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted);
 // threads per triplet work-group for a mean neighbour count kmean (GS_T3_WG overrides)
 int triplet_wg(double kmean);
 // apexes per queue fetch for n work items over nblocks work-groups: about 128 fetches per work-group
+// (GS_APEX_BATCH overrides)
 int triplet_batch(int64_t n, int nblocks);
 void combine3(const long double* S, int nq, long double Wd, long double Wr, bool with_n3, double* n2, double* n3);
 

@@ -126,6 +126,8 @@ int triplet_wg(double kmean)
 
 int triplet_batch(int64_t n, int nblocks)
 {
+    const char* ev = getenv("GS_APEX_BATCH");   // tuning override (unset or empty: automatic)
+    if (ev && *ev) return std::max(1, std::min(64, atoi(ev)));
     return (int)std::max<int64_t>(1, std::min<int64_t>(16, n / (128 * (int64_t)std::max(nblocks, 1))));
 }
 

@@ -218,6 +218,36 @@ def test_4pcf_config3_full_size_properties(eng, monkeypatch):
     assert a["stats"]["n_binned"] > 0
 
 
+@pytest.mark.parametrize("batch", ["1", "7"])
+def test_apex_batches(eng, monkeypatch, batch):
+    # apexes fetched from the work queue one at a time or seven at a time (the automatic size is 1
+    # for catalogues this small): RSD weighted 3PCF, equilateral 3PCF and 4PCF give the same counts,
+    # sums and statistics either way, and the 3PCF counts equal the oracle's
+    rng = np.random.default_rng(17)
+    d = C.clustered(4000, 100.0, 91)
+    r = C.uniform(8000, 100.0, 92)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(4000, 0.2, 93), np.ones(8000)])
+    buf = (rng.random(len(xyz)) < 0.1).astype(np.int32)
+    eng.set_catalogue(xyz, w, 4000, buf)
+    monkeypatch.setenv("GS_APEX_BATCH", "1")
+    base3 = eng.pcf3(2.0, 12.0, 4, nmu=3, precision="fp32")
+    base_e = eng.pcf3_equi(2.0, 12.0, 4, precision="fp32")
+    base4 = eng.pcf4(2.0, 8.0, 3)
+    monkeypatch.setenv("GS_APEX_BATCH", batch)
+    o3 = eng.pcf3(2.0, 12.0, 4, nmu=3, precision="fp32")
+    oe = eng.pcf3_equi(2.0, 12.0, 4, precision="fp32")
+    o4 = eng.pcf4(2.0, 8.0, 3)
+    for o, b in ((o3, base3), (oe, base_e), (o4, base4)):
+        assert np.array_equal(o["cnt"], b["cnt"])
+        assert o["stats"]["n_units"] == b["stats"]["n_units"]
+        assert o["stats"]["n_candidates"] == b["stats"]["n_candidates"]
+    # 3PCF weighted sums are exact fixed point; the 4PCF's weighted products are FP64 sums
+    assert np.array_equal(o3["wsum"], base3["wsum"]) and _close(o4["n2"], base4["n2"])
+    ref = O.pcf3(xyz, w, 4000, O.Params(2.0, 12.0, 4, 3, False), buf=buf)
+    assert np.array_equal(o3["cnt"], ref["cnt"])
+
+
 def test_3pcf_config3_density_slab(eng):
     # a 150^3 slab at the density of the 3PCF configuration (600k points in 500^3)
     d = C.clustered(5400, 150.0, 81)

@@ -215,3 +215,23 @@ def test_adjoint_isotropic(eng, log, buffered):
         err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
         print(f"isotropic adjoint {prec} log={log} buffered={buffered}: {err:.3g}", flush=True)
         assert err < 1e-10
+
+
+def test_adjoint_linearity_and_extreme_scales(eng):
+    # dL/dw is linear in (dL/dN2, dL/dN3): scaling them by s scales the gradient by s, also where the
+    # fixed-point scale of the FP32 kernel would leave 2^-1000 .. 2^1000 (s = 1e-300 takes the
+    # FP64-summing kernel) or sits near its far end (s = 1e290)
+    d, r, t, theta, side = _subset_c5(n_gal=800, n_ran=2400, seed=3)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([np.exp(t @ theta), np.ones(len(r))])
+    eng.set_catalogue(xyz, w, len(d))
+    rng = np.random.default_rng(9)
+    g2 = rng.normal(0, 1, (20, 10))
+    g3 = rng.normal(0, 1, (20, 10))
+    args = (0.0, 120.0, 20, 10, False, "fp32")
+    base = eng.adjoint2(g2, g3, *args)
+    for s in (1e-300, 1e290, -3.0):
+        g = eng.adjoint2(s * g2, s * g3, *args) / s
+        err = np.max(np.abs(g - base)) / np.max(np.abs(base))
+        print(f"scale {s:g}: {err:.3g}", flush=True)
+        assert err < 1e-12, s

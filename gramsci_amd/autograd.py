@@ -10,7 +10,6 @@ so gradients flow to anything w was computed from (e.g. w = exp(t @ theta)).
 """
 from __future__ import annotations
 
-import numpy as np
 import torch
 
 from .engine import Engine
@@ -35,6 +34,7 @@ class WeightedXi(torch.autograd.Function):
         n3 = torch.from_numpy(r["n3"]).to(w.device)
         ctx.save_for_backward(wd, n2, n3)
         ctx.eng, ctx.xyz, ctx.n_data, ctx.params, ctx.buf = eng, xyz, n_data, params, buf
+        ctx.generation = eng.generation
         return n2 / n3
 
     @staticmethod
@@ -42,12 +42,19 @@ class WeightedXi(torch.autograd.Function):
         wd, n2, n3 = ctx.saved_tensors
         eng = ctx.eng
         g = gxi.detach().to(torch.float64)
-        g2 = (g / n3).cpu().numpy()
-        g3 = (-g * n2 / (n3 * n3)).cpu().numpy()
-        # the context may have been used for another catalogue since the forward pass
-        eng.set_catalogue(ctx.xyz, wd, ctx.n_data, ctx.buf)
+        # dxi/dN2 = 1/N3, dxi/dN3 = -N2/N3^2.  A bin whose incoming gradient is exactly zero
+        # (e.g. masked out of the loss) contributes nothing, even where N3 = 0 makes its own
+        # derivatives non-finite; any other non-finite value is passed on, so the adjoint
+        # returns NaN gradients as for a loss that depends on an undefined xi.
+        zero = g == 0
+        g2 = torch.where(zero, torch.zeros_like(g), g / n3).cpu().numpy()
+        g3 = torch.where(zero, torch.zeros_like(g), -g * n2 / (n3 * n3)).cpu().numpy()
+        # the context may hold another catalogue since the forward pass
+        if eng.generation != ctx.generation:
+            eng.set_catalogue(ctx.xyz, wd, ctx.n_data, ctx.buf)
+            ctx.generation = eng.generation
         out = torch.empty_like(wd)
-        eng.adjoint2(np.nan_to_num(g2), np.nan_to_num(g3), out=out, **ctx.params)
+        eng.adjoint2(g2, g3, out=out, **ctx.params)
         return out, None, None, None, None
 
 

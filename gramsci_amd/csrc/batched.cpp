@@ -38,7 +38,7 @@ void batch_release(GsBatch& b)
     b.d_off.release(); b.d_row_base.release(); b.cell_of.release(); b.count.release(); b.cell_start.release();
     b.nch.release(); b.row_chunk.release(); b.scan_tmp.release(); b.queue.release(); b.orig.release();
     b.d_regions.release(); b.params.release(); b.pw.release(); b.pl.release(); b.flags.release(); b.out.release();
-    b.xyz.release(); b.w.release(); b.buf.release(); b.ovf.release();
+    b.xyz.release(); b.w.release(); b.buf.release(); b.ovf.release(); b.big.release();
     b.valid = false;
 }
 
@@ -127,7 +127,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     const bool grow = (size_t)n > B.cell_of.cap || (size_t)(cells + 1) > B.count.cap ||
                       (size_t)(cells + 1) > B.cell_start.cap || (size_t)(rows_tot + 1) > B.nch.cap ||
                       (size_t)(rows_tot + 1) > B.row_chunk.cap || (size_t)scan_scratch_size(scan_n) > B.scan_tmp.cap ||
-                      nout > B.out.cap || (kind == 3 && (size_t)(n + 1) > B.ovf.cap) || B.queue.cap < 32;
+                      nout > B.out.cap || (kind == 3 && (size_t)(2 * n + 3) > B.ovf.cap) || B.queue.cap < 32;
     if (grow) {
         for (auto& pl : B.plans) {
             if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
@@ -139,8 +139,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     if ((rc = B.cell_of.reserve(n)) || (rc = B.count.reserve(cells + 1)) || (rc = B.cell_start.reserve(cells + 1)) ||
         (rc = B.nch.reserve(rows_tot + 1)) || (rc = B.row_chunk.reserve(rows_tot + 1)) ||
         (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(32)) ||
-        (kind == 3 && (rc = B.ovf.reserve(n + 1))) ||
-        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n)) || (rc = B.flags.reserve(n)) ||
+        (kind == 3 && (rc = B.ovf.reserve(2 * n + 3))) ||
+        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n + kPad)) || (rc = B.flags.reserve(n + kPad)) ||
         (rc = B.out.reserve(nout)))
         return rc;
     // per-plan parameter blob: regions, bins, stencil rows (kept for the plan's lifetime)
@@ -222,10 +222,18 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     }
     // 3PCF in two captured passes: the first with a neighbour list sized for twice the largest
     // mean neighbour count of any region's box (apexes with more neighbours are queued), the
-    // second over the queued apexes with the largest list that fits (count read on the device)
+    // second over the queued apexes with the largest list that fits (count read on the device).
+    // Apexes with more neighbours than that are queued again (B.ovf[n+1] count, [n+2] largest
+    // neighbour count, apexes from n+3) for a pass with device-memory lists after the replay.
     TripArgs ta1 = ta, ta2 = ta;
     int nblocks2 = 0;
     if (kind == 3) {
+        ta1.ovf_n = B.ovf.ptr + n + 1;
+        ta1.ovf = B.ovf.ptr + n + 3;
+        ta1.ovf_kmax = B.ovf.ptr + n + 2;
+        ta2.ovf_n = ta1.ovf_n;
+        ta2.ovf = ta1.ovf;
+        ta2.ovf_kmax = ta1.ovf_kmax;
         const double shell = 4.0 / 3.0 * M_PI * (std::pow(p.rmax, 3) - std::pow(std::max(p.rmin, 0.0), 3));
         double kmean = 0.0;
         for (int r = 0; r < nreg; ++r) {
@@ -234,6 +242,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
             kmean = std::max(kmean, (double)(B.off[r + 1] - B.off[r]) * shell / vol);
         }
         ta1.wg = triplet_wg(kmean);
+        const char* evl = getenv("GS_T3_KMAX_LDS");     // test override: a shorter second-pass list
+        if (evl && *evl) ta.kmax = ta1.kmax = ta2.kmax = std::max(64, std::min(ta.kmax, atoi(evl)));
         int kmax1 = std::min(ta.kmax, std::max(128, (int)std::ceil(2.0 * kmean / 64.0) * 64));
         const char* ev = getenv("GS_T3_KMAX");      // tuning override (unset or empty: automatic)
         if (ev && *ev) kmax1 = std::max(64, std::min(ta.kmax, atoi(ev)));
@@ -260,6 +270,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.queue.ptr, 32, 0, v.stream);
     if (e == hipSuccess && nblocks2 > 0) e = launch_fill_u32(B.ovf.ptr, 1, 0, v.stream);
+    if (e == hipSuccess && kind == 3) e = launch_fill_u32(B.ovf.ptr + n + 1, 2, 0, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.out.ptr, (int64_t)nout * 2, 0, v.stream);
     if (e == hipSuccess)
         e = launch_cell_count_b(B.xyz.ptr, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
@@ -299,7 +310,15 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     P.mst = mst;
     P.kind = kind;
     P.nh = nh;
-    if (kind == 3) P.kmax = ta.kmax;
+    if (kind == 3) {
+        P.kmax = ta.kmax;
+        P.t3 = ta;              // tables, outputs and regions of the device-memory pass
+        P.t3_list = B.ovf.ptr + n + 3;
+        P.t3_cnt = B.ovf.ptr + n + 1;
+        P.nb = nb;
+        P.fp64 = fp64;
+        P.weighted = weighted;
+    }
     *out = &P;
     return GS_OK;
 }
@@ -307,10 +326,18 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
 static int run_plan(gs_ctx* c, GsBatch::Plan& P, float* ms)
 {
     CtxView v = view(c);
+    GsBatch& B = *v.batch;
     (void)hipEventRecord(v.ev[2], v.stream);
     hipError_t e = hipGraphLaunch(P.exec, v.stream);
+    if (e != hipSuccess) return hip_fail(e, "batched graph launch");
+    if (P.kind == 3) {
+        // apexes with more neighbours than any LDS list: lists in device memory (not captured)
+        int rc = triplet_glist_pass(P.t3, P.nb, P.fp64, P.weighted, P.t3_list, P.t3_cnt, B.queue.ptr + 24, B.big,
+                                    v.n_cu, v.stream, nullptr);
+        if (rc) return rc;
+    }
     (void)hipEventRecord(v.ev[3], v.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+    e = hipStreamSynchronize(v.stream);
     if (e != hipSuccess) return hip_fail(e, "batched graph launch");
     if (ms) (void)hipEventElapsedTime(ms, v.ev[2], v.ev[3]);
     return GS_OK;
@@ -512,7 +539,7 @@ extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
     hipError_t e = hipMemcpy(h.data(), B.out.ptr, nout * sizeof(unsigned long long), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(&err, B.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "batched result copy");
-    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", P->kmax);
+    if (err) return fail(GS_EHIP, "batched 3PCF: an apex's neighbour list overflowed with no later pass (internal error)");
     std::vector<long double> S(nh);
     for (int r = 0; r < nreg; ++r) {
         const unsigned long long* hc = h.data() + (size_t)r * nh;

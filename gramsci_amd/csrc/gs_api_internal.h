@@ -8,6 +8,7 @@
 
 #include "../../include/gramsci_hip.h"
 #include "gs_common.h"
+#include "gs_kernels.h"
 
 namespace gs {
 
@@ -30,6 +31,7 @@ struct GsExtra {
     DevBuf<int32_t> queue;
     DevBuf<double> dbl;
     DevBuf<int32_t> i32;
+    DevBuf<unsigned char> big;      // device-memory neighbour lists of the last 3PCF/4PCF pass
 };
 void extra_release(GsExtra& x);
 
@@ -44,7 +46,8 @@ struct GsBatch {
     std::vector<double> wmr;             // per-region largest weight (1 for unit weights)
     double wmax = 1.0, xmax = 0.0;
     DevBuf<int32_t> d_off, d_row_base, cell_of, count, cell_start, nch, row_chunk, scan_tmp, queue, orig;
-    DevBuf<int32_t> ovf;                 // 3PCF: overflow count + apexes left to the second pass
+    DevBuf<int32_t> ovf;                 // 3PCF: queued apexes of the second and the device-memory pass
+    DevBuf<unsigned char> big;           // 3PCF: device-memory neighbour lists
     DevBuf<unsigned char> d_regions, params;
     DevBuf<double4> pw;
     DevBuf<float4> pl;
@@ -66,6 +69,12 @@ struct GsBatch {
         int nh = 0;                      // triplet histogram entries per region
         int kmax = 0;                    // triplet neighbour capacity per apex
         DevBuf<unsigned char> blob;      // regions, bins and stencil rows read by the captured kernels
+        // 3PCF: arguments of the device-memory pass run after each replay (queued apexes, count)
+        TripArgs t3{};
+        const int32_t* t3_list = nullptr;
+        const int32_t* t3_cnt = nullptr;
+        int nb = 0;
+        bool fp64 = false, weighted = false;
     };
     std::vector<Plan> plans;
     std::vector<std::pair<std::string, int>> tuned;   // parameter key -> tuned cells per rmax
@@ -116,7 +125,6 @@ int default_cell_div(const gs_ctx* c, double rmax, int mode);
 // 3PCF helpers shared by the single and batched paths (pcf34.cpp)
 std::vector<uint16_t> table3(int nb);
 std::vector<uint8_t> normal_table(int nm);
-struct TripArgs;
 int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted);
 // threads per triplet work-group for a mean neighbour count kmean (GS_T3_WG overrides)
 int triplet_wg(double kmean);
@@ -124,5 +132,10 @@ int triplet_wg(double kmean);
 // (GS_APEX_BATCH overrides)
 int triplet_batch(int64_t n, int nblocks);
 void combine3(const long double* S, int nq, long double Wd, long double Wr, bool with_n3, double* n2, double* n3);
+// last 3PCF pass over the apexes queued in d_list (count d_cnt[0], largest neighbour count d_cnt[1]),
+// with neighbour lists in device memory (pcf34.cpp); *n_apex (may be NULL) receives the count
+int triplet_glist_pass(const TripArgs& base, int nb, bool fp64, bool weighted, const int32_t* d_list,
+                       const int32_t* d_cnt, int32_t* d_queue, DevBuf<unsigned char>& scratch, int n_cu,
+                       hipStream_t st, int* n_apex);
 
 }  // namespace gs

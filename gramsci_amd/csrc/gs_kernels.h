@@ -105,7 +105,16 @@ struct TripArgs {
     int32_t log_bins;               // selects the kernel variant (log or linear bin estimate)
     int32_t wg;                     // threads per work-group (one apex each): 64, 128 or 256 (0: 256)
     int32_t batch;                  // apexes taken per queue fetch (0: 1)
+    // neighbour counts beyond every LDS list: with ovf_kmax set, the largest neighbour count of a
+    // queued apex is kept there (atomic max); with glist set, each work-group keeps its apex's
+    // neighbour list in device memory at glist + blockIdx.x * glist_stride bytes (kmax entries)
+    // instead of LDS, and the histogram goes to global memory
+    int32_t* ovf_kmax;
+    void* glist;
+    int64_t glist_stride;
 };
+// bytes per neighbour of a device-memory neighbour list (glist) of the triplet kernel
+size_t triplet3_glist_entry_bytes(const TripArgs& a, bool fp64, bool weighted);
 size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);
 int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted);
@@ -139,10 +148,20 @@ struct QuadArgs {
     int32_t* ovf_n;
     const int32_t* items;
     int32_t batch;              // apexes taken per queue fetch (0: 1)
+    // as for the triplet kernel: with ovf_kmax set, the largest neighbour count of a queued apex is
+    // kept there; with glist set, each work-group keeps its apex's neighbour list (kmax entries, a
+    // power of two) at glist + blockIdx.x * glist_stride bytes of device memory, with no link matrix
+    // (the neighbour-neighbour bins are evaluated where they are needed)
+    int32_t* ovf_kmax;
+    void* glist;
+    int64_t glist_stride;
 };
-size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra);
+// bytes per neighbour of a device-memory neighbour list (glist) of the quad kernel: x, y, z, w,
+// key, permutation, packed bin, and the sort's staging copy of x, y, z, w and the packed bin
+constexpr size_t kQuadGlistEntryBytes = 4 * 8 + 3 * 4 + 4 * 8 + 4;
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool glist = false);
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);
-int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool weighted);
+int quad4_occupancy(const QuadArgs& a, int nb, bool weighted);
 
 // Arguments of the persistent 2PCF adjoint kernel (adjoint.hip).
 struct AdjArgs {

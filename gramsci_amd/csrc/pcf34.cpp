@@ -23,6 +23,7 @@ void extra_release(GsExtra& x)
     x.queue.release();
     x.dbl.release();
     x.i32.release();
+    x.big.release();
 }
 
 // bintable for the 3PCF (gramsci.F90:1459-1474), 0-based configuration of a
@@ -152,6 +153,54 @@ int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted)
     return GS_OK;
 }
 
+// The last pass of the 3PCF: apexes whose neighbours outnumber every LDS list were queued
+// (d_list, count d_cnt[0], largest neighbour count d_cnt[1]); they run here with their
+// neighbour lists in device memory, one list per work-group, and the histogram in global
+// memory.  Reads the count (synchronising the stream) and launches nothing if it is 0.
+// `base` carries the pass's tables and outputs; d_queue is 8 work counters it may clear.
+int triplet_glist_pass(const TripArgs& base, int nb, bool fp64, bool weighted, const int32_t* d_list,
+                       const int32_t* d_cnt, int32_t* d_queue, DevBuf<unsigned char>& scratch, int n_cu,
+                       hipStream_t st, int* n_apex)
+{
+    int32_t h[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "3PCF queued apex count");
+    if (n_apex) *n_apex = h[0];
+    if (h[0] <= 0) return GS_OK;
+    TripArgs a = base;
+    a.items = d_list;
+    a.n_items = h[0];
+    a.n_items_dev = nullptr;
+    a.ovf = nullptr;
+    a.ovf_n = nullptr;
+    a.ovf_kmax = nullptr;
+    a.kmax = (std::max(h[1], 2) + 3) / 4 * 4;       // every list holds its apex's neighbours
+    a.wg = kWG;
+    a.batch = 1;
+    a.queue = d_queue;
+    const size_t stride = (a.kmax * triplet3_glist_entry_bytes(a, fp64, weighted) + 255) / 256 * 256;
+    a.glist = (void*)d_list;        // selects the device-memory variant (LDS sizes and occupancy query)
+    // the LDS holds no list here: the histogram stays in LDS (replicated) where it fits
+    a.lds_hist = 1;
+    a.reps = 1;
+    const size_t nh = (size_t)a.ncfg * (a.nm > 1 ? a.nm : 1) * 4;
+    while (a.reps < 32 && nh * a.reps * 2 * 4 <= 16 * 1024) a.reps *= 2;
+    while (a.reps > 1 && triplet3_lds_bytes(a, nb, fp64, weighted) > 64 * 1024) a.reps /= 2;
+    if (triplet3_lds_bytes(a, nb, fp64, weighted) > 64 * 1024) a.lds_hist = 0;
+    const int occ = triplet3_occupancy(a, nb, fp64, weighted);
+    const size_t budget = (size_t)1 << 30;           // device memory for the lists of concurrent apexes
+    const int nblk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)h[0], (size_t)n_cu * occ, budget / stride}));
+    int rc = scratch.reserve((size_t)nblk * stride);
+    if (rc) return rc;
+    a.glist = scratch.ptr;
+    a.glist_stride = (int64_t)stride;
+    e = hipMemsetAsync(d_queue, 0, 8 * sizeof(int32_t), st);
+    if (e == hipSuccess) e = launch_triplet3(a, nb, fp64, weighted, nblk, st);
+    if (e != hipSuccess) return hip_fail(e, "3PCF device-memory pass");
+    return GS_OK;
+}
+
 // N2/N3 of the 3PCF from the class sums S[q][4] (q = configuration x mu bin):
 // N2 = sum over classes of (-1)^r S_r / (Wd^(3-r) Wr^r) (gramsci.F90:501, :507, :431, :437);
 // N3 = S_3 / Wr^3 (:498, :505, :434) unless with_n3 is false (the equilateral RSD
@@ -253,13 +302,20 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     // sampled apexes (+5%) and hands every apex with more neighbours to a second pass over
     // just those apexes, with the largest list that fits.  An apex is counted in exactly
     // one pass, so the sums are those of one pass.
+    // An apex whose neighbours outnumber even the second pass's list is queued once more, for a
+    // last pass with its list in device memory (triplet_glist_pass): no neighbour count is refused.
     const int kmax_full = a.kmax;
     int kmax1 = kmax_full;
     const int nsample = std::min(n, 8192);
     const int stride = nsample > 0 ? n / nsample : 1;
-    if ((rc = X.i32.reserve((size_t)n + 1 + nsample))) return rc;
+    // i32: [0] second-pass count, [1, n] second-pass apexes, [n+1] last-pass count, [n+2] its largest
+    // neighbour count, [n+3, 2n+2] last-pass apexes, then the sampled neighbour counts
+    if ((rc = X.i32.reserve(2 * (size_t)n + 3 + nsample))) return rc;
+    int32_t* d_list2 = X.i32.ptr + 1;
+    int32_t* d_cnt3 = X.i32.ptr + n + 1;
+    int32_t* d_list3 = X.i32.ptr + n + 3;
     if (nsample > 0) {
-        int32_t* d_cnt = X.i32.ptr + n + 1;
+        int32_t* d_cnt = X.i32.ptr + 2 * n + 3;
         std::vector<int32_t> kc(nsample);
         e = launch_nbr_count(a, stride, nsample, d_cnt, v.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(kc.data(), d_cnt, nsample * sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
@@ -281,16 +337,28 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
         const char* ev = getenv("GS_T3_KMAX");      // tuning override (unset or empty: automatic)
         if (ev && *ev) kmax1 = std::max(64, std::min(kmax_full, atoi(ev)));
     }
-    const bool two_pass = kmax1 < kmax_full;
+    {
+        const char* ev = getenv("GS_T3_KMAX_LDS");  // test override: a shorter second-pass list
+        if (ev && *ev) {
+            a.kmax = std::max(64, std::min(kmax_full, atoi(ev)));
+            kmax1 = std::min(kmax1, a.kmax);
+        }
+    }
+    const bool two_pass = kmax1 < a.kmax;
     (void)hipEventRecord(v.ev[2], v.stream);
     if (n > 0) {
+        e = hipMemsetAsync(X.i32.ptr, 0, sizeof(int32_t), v.stream);
+        if (e == hipSuccess) e = hipMemsetAsync(d_cnt3, 0, 2 * sizeof(int32_t), v.stream);
+        if (e != hipSuccess) return hip_fail(e, "3PCF overflow counters");
         TripArgs a1 = a;
+        a1.ovf_kmax = d_cnt3 + 1;
         if (two_pass) {
             a1.kmax = kmax1;
-            a1.ovf = X.i32.ptr + 1;
+            a1.ovf = d_list2;
             a1.ovf_n = X.i32.ptr;
-            e = hipMemsetAsync(X.i32.ptr, 0, sizeof(int32_t), v.stream);
-            if (e != hipSuccess) return hip_fail(e, "3PCF overflow counter");
+        } else {
+            a1.ovf = d_list3;
+            a1.ovf_n = d_cnt3;
         }
         const int occ = triplet3_occupancy(a1, nb, fp64, weighted);
         const int nblk1 = std::max(1, std::min(v.n_cu * occ, n));
@@ -304,10 +372,13 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
             if (e != hipSuccess) return hip_fail(e, "3PCF first pass");
             if (nov > 0) {
                 TripArgs a2 = a;
-                a2.items = X.i32.ptr + 1;
+                a2.items = d_list2;
                 a2.n_items = nov;
                 a2.wg = kWG;            // the apexes with the most neighbours, one per fetch
                 a2.batch = 1;
+                a2.ovf = d_list3;
+                a2.ovf_n = d_cnt3;
+                a2.ovf_kmax = d_cnt3 + 1;
                 e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);   // work counters only
                 const int occ2 = triplet3_occupancy(a2, nb, fp64, weighted);
                 if (e == hipSuccess)
@@ -315,6 +386,9 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
                 if (e != hipSuccess) return hip_fail(e, "3PCF second pass");
             }
         }
+        if ((rc = triplet_glist_pass(a, nb, fp64, weighted, d_list3, d_cnt3, X.queue.ptr, X.big, v.n_cu, v.stream,
+                                     nullptr)))
+            return rc;
     }
     (void)hipEventRecord(v.ev[3], v.stream);
     std::vector<unsigned long long> h_out(nout);
@@ -323,7 +397,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     if (e == hipSuccess) e = hipMemcpyAsync(&err, X.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
     if (e != hipSuccess) return hip_fail(e, "3PCF kernel");
-    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", kmax_full);
+    if (err) return fail(GS_EHIP, "3PCF: an apex's neighbour list overflowed with no later pass (internal error)");
 
     std::vector<long double> S(nh);
     for (int k = 0; k < nh; ++k) {
@@ -381,7 +455,11 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
     std::vector<uint16_t> tab = tetra ? std::vector<uint16_t>() : table4(nb);
 
-    const size_t budget = 64 * 1024;
+    // LDS plan of the second pass: the longest list (a power of two, 64..256, with its K x K link
+    // matrix) that fits gfx950's 160 KiB of LDS per work-group beside the tables and, where it
+    // fits in 16 KiB, the count histogram.  Apexes with more neighbours go to a last pass with
+    // device-memory lists, so no neighbour count is refused.
+    const size_t budget = 160 * 1024;
     int kmax = 256;
     bool lds_hist = (size_t)ncell * 64 <= 16 * 1024;
     while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) kmax /= 2;
@@ -392,6 +470,11 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     }
     if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget)
         return fail(GS_ELIMIT, "4PCF tables do not fit in LDS");
+    {
+        const char* ev = getenv("GS_T4_KMAX_LDS");  // test override: a shorter second-pass list
+        if (ev && *ev)
+            while (kmax > 64 && kmax > atoi(ev)) kmax /= 2;
+    }
 
     GsExtra& X = *v.extra;
     const size_t tab_off = ((sizeof(BinDesc) + 15) / 16) * 16;
@@ -442,7 +525,12 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     int kmax1 = kmax;
     const int nsample = std::min(n, 8192);
     const int stride = nsample > 0 ? n / nsample : 1;
-    if ((rc = X.i32.reserve((size_t)n + 1 + nsample))) return rc;
+    // i32 as for the 3PCF: [0] second-pass count, [1, n] its apexes, [n+1] last-pass count, [n+2]
+    // its largest neighbour count, [n+3, 2n+2] its apexes, then the sampled neighbour counts
+    if ((rc = X.i32.reserve(2 * (size_t)n + 3 + nsample))) return rc;
+    int32_t* d_list2 = X.i32.ptr + 1;
+    int32_t* d_cnt3 = X.i32.ptr + n + 1;
+    int32_t* d_list3 = X.i32.ptr + n + 3;
     if (nsample > 0 && kmax > 64) {
         TripArgs ts{};
         ts.pw = a.pw;
@@ -452,7 +540,7 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
         ts.bins = a.bins;
         ts.g = a.g;
         ts.nrows = a.nrows;
-        int32_t* d_cnt = X.i32.ptr + n + 1;
+        int32_t* d_cnt = X.i32.ptr + 2 * n + 3;
         std::vector<int32_t> kc(nsample);
         e = launch_nbr_count(ts, stride, nsample, d_cnt, v.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(kc.data(), d_cnt, nsample * sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
@@ -479,15 +567,20 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     const bool two_pass = kmax1 < kmax;
     (void)hipEventRecord(v.ev[2], v.stream);
     if (n > 0) {
+        e = hipMemsetAsync(X.i32.ptr, 0, sizeof(int32_t), v.stream);
+        if (e == hipSuccess) e = hipMemsetAsync(d_cnt3, 0, 2 * sizeof(int32_t), v.stream);
+        if (e != hipSuccess) return hip_fail(e, "4PCF overflow counters");
         QuadArgs a1 = a;
+        a1.ovf_kmax = d_cnt3 + 1;
         if (two_pass) {
             a1.kmax = kmax1;
             a1.ovf_n = X.i32.ptr;
-            a1.ovf = X.i32.ptr + 1;
-            e = hipMemsetAsync(X.i32.ptr, 0, sizeof(int32_t), v.stream);
-            if (e != hipSuccess) return hip_fail(e, "4PCF overflow counter");
+            a1.ovf = d_list2;
+        } else {
+            a1.ovf_n = d_cnt3;
+            a1.ovf = d_list3;
         }
-        const int occ = quad4_occupancy(nb, ncell, a1.kmax, lds_hist, tetra, weighted);
+        const int occ = quad4_occupancy(a1, nb, weighted);
         const int nblk1 = std::max(1, std::min(v.n_cu * occ, n));
         a1.batch = triplet_batch(n, nblk1);
         e = launch_quad4(a1, nb, weighted, nblk1, v.stream);
@@ -499,15 +592,44 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
             if (e != hipSuccess) return hip_fail(e, "4PCF first pass");
             if (nov > 0) {
                 QuadArgs a2 = a;
-                a2.items = X.i32.ptr + 1;
+                a2.items = d_list2;
                 a2.n_items = nov;
                 a2.batch = 1;
+                a2.ovf = d_list3;
+                a2.ovf_n = d_cnt3;
+                a2.ovf_kmax = d_cnt3 + 1;
                 e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);   // work counters only
-                const int occ2 = quad4_occupancy(nb, ncell, kmax, lds_hist, tetra, weighted);
+                const int occ2 = quad4_occupancy(a2, nb, weighted);
                 if (e == hipSuccess)
                     e = launch_quad4(a2, nb, weighted, std::max(1, std::min(v.n_cu * occ2, nov)), v.stream);
                 if (e != hipSuccess) return hip_fail(e, "4PCF second pass");
             }
+        }
+        // last pass: the apexes whose neighbours outnumber the second pass's list, with their lists
+        // in device memory (one per work-group; no link matrix)
+        int32_t h3[2] = {0, 0};
+        e = hipMemcpyAsync(h3, d_cnt3, sizeof(h3), hipMemcpyDeviceToHost, v.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
+        if (e != hipSuccess) return hip_fail(e, "4PCF queued apex count");
+        if (h3[0] > 0) {
+            QuadArgs a3 = a;
+            a3.items = d_list3;
+            a3.n_items = h3[0];
+            a3.batch = 1;
+            a3.kmax = 64;
+            while (a3.kmax < h3[1]) a3.kmax *= 2;           // a power of two for the bitonic sort
+            const size_t gstride = (a3.kmax * kQuadGlistEntryBytes + 255) / 256 * 256;
+            a3.glist = (void*)d_list3;      // selects the device-memory variant for the occupancy query
+            const int occ3 = quad4_occupancy(a3, nb, weighted);
+            const size_t gbudget = (size_t)1 << 30;
+            const int nblk3 = (int)std::max<size_t>(1, std::min<size_t>({(size_t)h3[0], (size_t)v.n_cu * occ3,
+                                                                      gbudget / gstride}));
+            if ((rc = X.big.reserve((size_t)nblk3 * gstride))) return rc;
+            a3.glist = X.big.ptr;
+            a3.glist_stride = (int64_t)gstride;
+            e = hipMemsetAsync(X.queue.ptr, 0, 8 * sizeof(int32_t), v.stream);
+            if (e == hipSuccess) e = launch_quad4(a3, nb, weighted, nblk3, v.stream);
+            if (e != hipSuccess) return hip_fail(e, "4PCF device-memory pass");
         }
     }
     (void)hipEventRecord(v.ev[3], v.stream);
@@ -520,7 +642,7 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     if (e == hipSuccess) e = hipMemcpyAsync(&err, X.queue.ptr + 8, sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
     if (e != hipSuccess) return hip_fail(e, "4PCF kernel");
-    if (err) return fail(GS_ELIMIT, "an apex has more than %d neighbours in (rmin, rmax); use a smaller rmax", kmax);
+    if (err) return fail(GS_EHIP, "4PCF: an apex's neighbour list overflowed with no later pass (internal error)");
 
     if (cnt) memcpy(cnt, h_out.data(), ncnt * sizeof(uint64_t));
     // normalised weight of one point: +w/Wd (data) or -w/Wr (random) (gramsci.F90:1390-1391)

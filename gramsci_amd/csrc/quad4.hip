@@ -31,18 +31,22 @@ namespace gs {
 
 using namespace dev;
 
-size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra)
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool glist)
 {
     size_t b = 0;
     if (lds_hist) b += (size_t)ncell * 16 * 4;
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;            // t64
     if (!tetra) b += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;   // perimeter table (u16)
+    if (glist) return b;                                         // neighbour list in device memory
     b += (size_t)kmax * (24 + 8 + 4 + 4 + 4);                     // x, y, z, w, key, slot/packed, sorted j
     b += ((size_t)kmax * kmax + 15) / 16 * 16;                    // link matrix (u8: bin+1, 0 = not kept)
     return b;
 }
 
-template <bool W>
+// BIG: the apex's neighbour list in device memory (a.glist), for apexes with more neighbours
+// than an LDS list holds; the neighbour-neighbour bins are evaluated where needed (no K x K
+// matrix), and the quads run in segments with the LDS counts drained after each.
+template <bool W, bool BIG>
 __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -61,6 +65,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
     uint16_t* tab = (uint16_t*)p;
     if (!tetra) p += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;
+    if constexpr (BIG) p = (unsigned char*)a.glist + (size_t)blockIdx.x * (size_t)a.glist_stride;
     double* nx_ = (double*)p;
     double* ny_ = nx_ + kmax;
     double* nz_ = ny_ + kmax;
@@ -68,7 +73,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     int32_t* key = (int32_t*)(nw_ + kmax);   // original index (sort key)
     int32_t* pk = key + kmax;                // sort permutation, then bin | rand << 8 in sorted order
     int32_t* jl = pk + kmax;                 // bin | rand << 8 in gather order
-    uint8_t* L = (uint8_t*)(jl + kmax);      // L[u * kmax + v]
+    uint8_t* L = (uint8_t*)(jl + kmax);      // L[u * kmax + v] (LDS list; BIG: the sort's staging area)
 
     __shared__ int s_item, s_k, s_over;
 
@@ -85,6 +90,18 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     const int xcd = blockIdx.x & 7;
     int tried = 0;
     const int nb2 = nb * nb;
+
+    // adds the LDS counts to the global counts and clears them (all threads of the work-group)
+    auto drain_counts = [&]() {
+        for (int k = tid; k < ncell * 16; k += kWG) {
+            const uint32_t v = hc[k];
+            if (v) {
+                atomicAdd(&a.out_cnt[k], (unsigned long long)v);
+                hc[k] = 0;
+            }
+        }
+    };
+    unsigned long long bound = 0;   // LDS count increments since the last drain (work-group uniform)
 
     const int bsz = max(a.batch, 1);
     int b_next = 0, b_end = 0;      // thread 0: the work-group's current batch of apexes
@@ -144,9 +161,12 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         const int K = min(s_k, kmax);
         if (s_over) {
             if (a.ovf) {
-                // left to the second pass, which has room for all its neighbours
-                if (tid == 0) a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
-                cand = cand0;       // the second pass counts this apex's candidates
+                // left to a later pass with room for all its neighbours (s_k counted every one)
+                if (tid == 0) {
+                    a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
+                    if (a.ovf_kmax) atomicMax(a.ovf_kmax, s_k);
+                }
+                cand = cand0;       // the later pass counts this apex's candidates
                 over = over0;
                 __syncthreads();
                 continue;
@@ -177,8 +197,9 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
                 __syncthreads();
             }
         }
-        // gather into sorted order: slot pk[u] -> position u (stage through the link-matrix area)
-        double* tmp = (double*)L;   // L is not in use yet; needs 5*K*8 bytes <= kmax*kmax
+        // gather into sorted order: slot pk[u] -> position u, staged through the link-matrix area
+        // (not in use yet; 5*K*8 <= kmax*kmax bytes) or a device-memory list's staging area
+        double* tmp = (double*)L;
         for (int u = tid; u < K; u += kWG) {
             const int sl = pk[u];
             tmp[u] = nx_[sl];
@@ -194,7 +215,17 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         }
         __syncthreads();
 
-        // ---- 3. link matrix of neighbour-neighbour side bins (find_dist)
+        // ---- 3. link matrix of neighbour-neighbour side bins (find_dist); a device-memory list
+        // evaluates them where they are needed instead
+        auto link = [&](int x, int y) -> int {       // x < y: bin + 1 of side (x, y), 0 if not kept
+            if constexpr (BIG) {
+                const int b = bin_of(exact_s(nx_[x], ny_[x], nz_[x], nx_[y], ny_[y], nz_[y]), t64, nb);
+                return (b >= 0 && b < nb) ? b + 1 : 0;
+            } else {
+                return L[x * K + y];
+            }
+        };
+        if (!BIG)
         for (int e = tid; e < K * K; e += kWG) {
             const int u = e / K, v = e - u * K;
             uint8_t val = 0;
@@ -213,18 +244,29 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         int u = 0;
         long long rem = tid;
         while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
-        for (long long pi = tid; pi < P; pi += kWG) {
+        // one segment for an LDS list; for a device-memory list segments of at most 2^30 / K pairs
+        // (each adds fewer than K counts), with the u32 LDS counts drained after each
+        const long long seg = BIG ? max((long long)kWG, ((1ll << 30) / max(K, 1)) / kWG * kWG) : max(P, 1ll);
+        long long pi = tid;
+        for (long long s0 = 0; s0 < P; s0 += seg) {
+        if (BIG && s0 > 0) {
+            __syncthreads();
+            if (lh) drain_counts();
+            __syncthreads();
+        }
+        const long long s1 = min(P, s0 + seg);
+        for (; pi < s1; pi += kWG) {
             const int v = u + 1 + (int)rem;
-            const int ind4 = L[u * K + v];
+            const int ind4 = link(u, v);
             const int p1 = pk[u], p2 = pk[v];
             const int ind1 = (p1 & 255) + 1, ind2 = (p2 & 255) + 1;
             if (ind4 && (!tetra || (ind2 == ind1 && ind4 == ind1))) {     // tetra: gramsci.F90:564-567
                 units += (unsigned long long)(K - 1 - v);
                 const int r1 = (p1 >> 8) & 1, r2 = (p2 >> 8) & 1;
                 for (int k3 = v + 1; k3 < K; ++k3) {
-                    const int ind5 = L[v * K + k3];
+                    const int ind5 = link(v, k3);
                     if (!ind5) continue;
-                    const int ind6 = L[u * K + k3];
+                    const int ind6 = link(u, k3);
                     if (!ind6) continue;
                     const int p3 = pk[k3];
                     const int ind3 = (p3 & 255) + 1;
@@ -266,12 +308,23 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             rem += kWG;
             while (u < K - 1 && rem >= K - 1 - u) { rem -= K - 1 - u; ++u; }
         }
+        }
         __syncthreads();
+        // The u32 LDS counts take at most C(K,3) increments per apex (K <= 256 in LDS, so fewer
+        // than 2^26); the work-group drains them into the 64-bit global counts before the
+        // increments since the last drain could pass 2^31 (a persistent work-group may count many
+        // apexes of hundreds of neighbours).  A device-memory list drains after every apex.
+        if (lh) {
+            bound += BIG ? (1ull << 31) : (unsigned long long)K * (K - 1) * (K - 2) / 6;
+            if (bound > (1ull << 31) - (1ull << 26)) {
+                drain_counts();
+                bound = 0;
+                __syncthreads();
+            }
+        }
     }
     __syncthreads();
-    if (lh)
-        for (int k = tid; k < ncell * 16; k += kWG)
-            if (hc[k]) atomicAdd(&a.out_cnt[k], (unsigned long long)hc[k]);
+    if (lh) drain_counts();
     for (int off = 32; off > 0; off >>= 1) {
         units += __shfl_down(units, off, 64);
         binned += __shfl_down(binned, off, 64);
@@ -286,10 +339,10 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     }
 }
 
-template <bool W>
+template <bool W, bool BIG>
 static hipError_t launch_q(const QuadArgs& a, size_t lds, int nblocks, hipStream_t st)
 {
-    auto kern = k_quad4<W>;
+    auto kern = k_quad4<W, BIG>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -300,16 +353,23 @@ static hipError_t launch_q(const QuadArgs& a, size_t lds, int nblocks, hipStream
 
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st)
 {
-    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0);
-    return weighted ? launch_q<true>(a, lds, nblocks, st) : launch_q<false>(a, lds, nblocks, st);
+    const bool big = a.glist != nullptr;
+    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0, big);
+    if (big) return weighted ? launch_q<true, true>(a, lds, nblocks, st) : launch_q<false, true>(a, lds, nblocks, st);
+    return weighted ? launch_q<true, false>(a, lds, nblocks, st) : launch_q<false, false>(a, lds, nblocks, st);
 }
 
-int quad4_occupancy(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool weighted)
+int quad4_occupancy(const QuadArgs& a, int nb, bool weighted)
 {
-    const size_t lds = quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra);
+    const bool big = a.glist != nullptr;
+    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0, big);
+    void (*kern)(QuadArgs) = big ? (weighted ? k_quad4<true, true> : k_quad4<false, true>)
+                                 : (weighted ? k_quad4<true, false> : k_quad4<false, false>);
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 1;
     int n = 0;
-    hipError_t e = weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_quad4<true>, kWG, lds)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_quad4<false>, kWG, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kWG, lds);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 

@@ -26,12 +26,19 @@
 //      evaluated directly from the neighbours' coordinates - in FP64, or in
 //      FP32 relative to the apex with the same guard band and FP64 recheck as
 //      the pair kernel - instead of a binary search in a neighbour list.
+#include <type_traits>
+
 #include "gs_device.h"
 #include "gs_kernels.h"
 
 namespace gs {
 
 using namespace dev;
+
+size_t triplet3_glist_entry_bytes(const TripArgs& a, bool fp64, bool weighted)
+{
+    return (fp64 ? 32 : 16) + 4 + (weighted ? 8 : 0) + ((a.nm > 1 || weighted) ? 4 : 0);
+}
 
 size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
 {
@@ -46,8 +53,8 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;           // t64
     b += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;             // config table (u16)
     if (a.nm > 1) b += ((size_t)a.nm * a.nm + 15) / 16 * 16;    // find_normal table (u8)
-    const size_t rec = fp64 ? 32 : 16;                          // coords + packed bin/class/mu
-    b += kmax * (rec + 4 + (weighted ? 8 : 0) + ((a.nm > 1 || weighted) ? 4 : 0));   // + sorted index, weight, orig
+    // neighbour list: coords + packed bin/class/mu, sorted index, weight, orig (in LDS unless glist)
+    if (!a.glist) b += kmax * triplet3_glist_entry_bytes(a, fp64, weighted);
     return b;
 }
 
@@ -57,10 +64,11 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
 // The variant is a template parameter so that each one pays only for its own
 // registers: MODE bit 0 RSD (nm >= 2), bit 1 equilateral (0 = plain); GEN 1 for
 // a batched launch or a histogram in global memory (the plain launch with an
-// LDS histogram, the common case, runs with GEN 0).
+// LDS histogram, the common case, runs with GEN 0), GEN 2 for neighbour lists in
+// device memory (apexes with more neighbours than an LDS list holds).
 enum { kT3Rsd = 1, kT3Equi = 2 };
 
-template <bool FP64, bool W, int MODE, bool GEN, bool LOGB>
+template <bool FP64, bool W, int MODE, int GEN, bool LOGB>
 __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -94,6 +102,8 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     p += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;
     uint8_t* ntab = (uint8_t*)p;
     if (nm > 1) p += ((size_t)nm * nm + 15) / 16 * 16;
+    // the neighbour list: in LDS, or (GEN 2) in this work-group's part of device memory
+    if constexpr (GEN == 2) p = (unsigned char*)a.glist + (size_t)blockIdx.x * (size_t)a.glist_stride;
     float4* rec32 = (float4*)p;          // FP32: x, y, z relative to the apex, packed bin/class/mu
     double4* rec64 = (double4*)p;        // FP64: absolute x, y, z, packed bin/class/mu (as a double)
     p += (size_t)kmax * (FP64 ? 32 : 16);
@@ -254,16 +264,21 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         const int K = min(s_k, kmax);
         if (s_over) {
             if (a.ovf) {
-                // left to the second pass, which has room for all its neighbours
-                if (tid == 0) a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
-                cand = cand0;       // the second pass counts this apex's candidates
+                // left to a later pass with room for all its neighbours (s_k counted every one)
+                if (tid == 0) {
+                    a.ovf[atomicAdd(a.ovf_n, 1)] = apex;
+                    if (a.ovf_kmax) atomicMax(a.ovf_kmax, s_k);
+                }
+                cand = cand0;       // the later pass counts this apex's candidates
                 over = over0;
                 __syncthreads();
                 continue;
             }
             if (tid == 0) atomicExch(a.err, 1);
         }
-        const int P = K * (K - 1) / 2;          // K <= 4096, so P < 2^31
+        // pair index: K <= 4096 in LDS, so P < 2^31; a device-memory list may hold more
+        using pidx_t = std::conditional_t<GEN == 2, long long, int>;
+        const pidx_t P = (pidx_t)K * (K - 1) / 2;
         if (tid == 0) units += (unsigned long long)P;
         const int ra = (int)(fa & 1u);
         const double wa = Pa.w;
@@ -276,7 +291,20 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         const int Kc = max(K, 1);
         const int dq = nt / Kc, dr = nt - dq * Kc;
         int r = tid / Kc, c = tid - r * Kc;
-        for (int pi = tid; pi < P; pi += nt) {
+        // The pairs run in segments: one for an LDS list; for a device-memory list (whose apex
+        // may have so many neighbours that its pairs alone could overflow a u32 LDS count)
+        // segments of 2^30 pairs, with the LDS histogram drained after each.
+        constexpr pidx_t kSeg = GEN == 2 ? ((pidx_t)1 << 30) : (pidx_t)0;
+        pidx_t pi = tid;
+        for (pidx_t seg0 = 0; seg0 < P; seg0 += (GEN == 2 ? kSeg : P)) {
+        const pidx_t seg1 = GEN == 2 ? min(P, seg0 + kSeg) : P;
+        if (GEN == 2 && seg0 > 0) {
+            __syncthreads();
+            if (lh) drain(cur_r);
+            bound = 0;
+            __syncthreads();
+        }
+        for (; pi < seg1; pi += nt) {
             const bool first = c < K - 1 - r;
             const int u = first ? r : K - 2 - r;
             const int v = first ? r + 1 + c : c;
@@ -359,7 +387,8 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             c -= carry ? Kc : 0;
             r += dq + (carry ? 1 : 0);
         }
-        bound += (unsigned long long)P;
+        }
+        bound += (unsigned long long)(GEN == 2 ? (P - (P - 1) / kSeg * kSeg) : P);   // pairs since the last drain
         __syncthreads();
         if (lh && bound > (1ull << 31)) {
             drain(cur_r);
@@ -388,15 +417,16 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
 using T3Kernel = void (*)(TripArgs);
 
 template <bool FP64, bool W, int MODE, bool LOGB>
-static T3Kernel pick_t3m(bool gen)
+static T3Kernel pick_t3m(int gen)
 {
-    return gen ? k_triplet3<FP64, W, MODE, true, LOGB> : k_triplet3<FP64, W, MODE, false, LOGB>;
+    return gen == 2 ? k_triplet3<FP64, W, MODE, 2, LOGB>
+                    : (gen ? k_triplet3<FP64, W, MODE, 1, LOGB> : k_triplet3<FP64, W, MODE, 0, LOGB>);
 }
 
 template <bool FP64, bool W, bool LOGB>
 static T3Kernel pick_t3w(const TripArgs& a)
 {
-    const bool gen = a.nreg > 0 || !a.lds_hist;
+    const int gen = a.glist ? 2 : ((a.nreg > 0 || !a.lds_hist) ? 1 : 0);
     if (a.equi)
         return a.nm > 1 ? pick_t3m<FP64, W, kT3Equi | kT3Rsd, LOGB>(gen) : pick_t3m<FP64, W, kT3Equi, LOGB>(gen);
     return a.nm > 1 ? pick_t3m<FP64, W, kT3Rsd, LOGB>(gen) : pick_t3m<FP64, W, 0, LOGB>(gen);

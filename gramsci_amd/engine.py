@@ -38,6 +38,7 @@ class Engine:
         L.check(self._lib.gs_create(ctypes.byref(self._ctx), int(device)))
         self.n_data = 0
         self.n_rand = 0
+        self.generation = 0     # incremented by every catalogue upload
 
     def close(self):
         if self._ctx:
@@ -93,6 +94,7 @@ class Engine:
             L.check(self._lib.gs_set_catalogue(self._ctx, L.ptr(xyz), L.ptr(w), L.ptr(buf, ctypes.c_int32),
                                                nd, n - nd))
         self.n_data, self.n_rand = nd, n - nd
+        self.generation += 1
 
     def set_catalogue_batched(self, xyz, w, buf, off, ndata):
         """Independent regions: region r is points off[r]:off[r+1] (data first,
@@ -106,6 +108,7 @@ class Engine:
                                                    L.ptr(off, ctypes.c_int64), L.ptr(ndata, ctypes.c_int64),
                                                    len(ndata)))
         self.nreg = len(ndata)
+        self.generation += 1
 
     def decompose(self, xyz, region, rmax, nreg=None):
         """Domain decomposition on the GPU (gs_decompose): labels 1..nreg (default

@@ -332,7 +332,8 @@ contains
     close(11)
   end subroutine read_cut
 
-  ! raw (un-normalised) sum for -cut mode: N2 is sum(w_i w_j ...) over the data
+  ! raw (un-normalised) sums of the 4PCF in -cut mode, whose entry points return the normalised
+  ! sums only: N2 = sum(w_i w_j w_k w_l) over the data = normalised N2 * (sum w)^4
   real(c_double) function cut_scale(k) result(s)
     integer, intent(in) :: k
     real(c_double) :: sw
@@ -349,14 +350,16 @@ contains
 
   subroutine run_2pcf()
     integer(c_int64_t), allocatable, target :: cnt(:,:,:)
-    real(c_double), allocatable, target :: n2(:,:), n3(:,:)
+    real(c_double), allocatable, target :: n2(:,:), n3(:,:), ws(:,:,:)
     type(gs_stats) :: st
     integer :: l, k
-    allocate(cnt(nmu, nbins, 3), n2(nmu, nbins), n3(nmu, nbins))
-    call check(gs_count_2pcf(ctx, prm, c_loc(cnt), c_null_ptr, c_loc(n2), c_loc(n3), st), 'gs_count_2pcf')
+    allocate(cnt(nmu, nbins, 3), n2(nmu, nbins), n3(nmu, nbins), ws(nmu, nbins, 3))
+    call check(gs_count_2pcf(ctx, prm, c_loc(cnt), c_loc(ws), c_loc(n2), c_loc(n3), st), 'gs_count_2pcf')
     call report(st, '2PCF')
     if (cut_mode) then
-       n2 = n2*cut_scale(2)
+       ! every point is data with its weight as read: N2 is the raw data-data sum, which the
+       ! engine returns exactly (class 1 of the weight sums)
+       n2 = ws(:, :, 1)
        n3 = 0.0d0
     end if
     ! gramsci.F90:382-397
@@ -373,15 +376,15 @@ contains
 
   subroutine run_3pcf()
     integer(c_int64_t), allocatable, target :: cnt(:,:,:)
-    real(c_double), allocatable, target :: n2(:,:), n3(:,:)
+    real(c_double), allocatable, target :: n2(:,:), n3(:,:), ws(:,:,:)
     type(gs_stats) :: st
     integer :: i1, i2, i3, l, cfg, ncfg
     ncfg = nbins*(nbins + 1)*(nbins + 2)/6
-    allocate(cnt(4, nmu, ncfg), n2(nmu, ncfg), n3(nmu, ncfg))
-    call check(gs_count_3pcf(ctx, prm, c_loc(cnt), c_null_ptr, c_loc(n2), c_loc(n3), st), 'gs_count_3pcf')
+    allocate(cnt(4, nmu, ncfg), n2(nmu, ncfg), n3(nmu, ncfg), ws(4, nmu, ncfg))
+    call check(gs_count_3pcf(ctx, prm, c_loc(cnt), c_loc(ws), c_loc(n2), c_loc(n3), st), 'gs_count_3pcf')
     call report(st, '3PCF')
     if (cut_mode) then
-       n2 = n2*cut_scale(3)
+       n2 = ws(1, :, :)      ! raw data-data-data sum (all points are data)
        n3 = 0.0d0
     end if
     ! gramsci.F90:513-543, configurations in i <= j <= k order; with nmu >= 2 one row per mu bin
@@ -415,15 +418,15 @@ contains
   ! -equi: triangles whose three sides share one bin (gramsci.F90:402-461); no header line
   subroutine run_equi()
     integer(c_int64_t), allocatable, target :: cnt(:,:,:)
-    real(c_double), allocatable, target :: n2(:,:), n3(:,:)
+    real(c_double), allocatable, target :: n2(:,:), n3(:,:), ws(:,:,:)
     type(gs_stats) :: st
     integer :: l, k
-    allocate(cnt(4, nmu, nbins), n2(nmu, nbins), n3(nmu, nbins))
-    call check(gs_count_3pcf_equi(ctx, prm, c_loc(cnt), c_null_ptr, c_loc(n2), c_loc(n3), st), &
+    allocate(cnt(4, nmu, nbins), n2(nmu, nbins), n3(nmu, nbins), ws(4, nmu, nbins))
+    call check(gs_count_3pcf_equi(ctx, prm, c_loc(cnt), c_loc(ws), c_loc(n2), c_loc(n3), st), &
          'gs_count_3pcf_equi')
     call report(st, 'equilateral 3PCF')
     if (cut_mode) then
-       n2 = n2*cut_scale(3)
+       n2 = ws(1, :, :)      ! raw data-data-data sum (all points are data)
        n3 = 0.0d0
     end if
     open(11, file=trim(file_out), status='unknown')

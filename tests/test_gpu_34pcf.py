@@ -218,11 +218,14 @@ def test_4pcf_config3_full_size_properties(eng, monkeypatch):
     assert a["stats"]["n_binned"] > 0
 
 
-@pytest.mark.parametrize("batch", ["1", "7"])
-def test_apex_batches(eng, monkeypatch, batch):
-    # apexes fetched from the work queue one at a time or seven at a time (the automatic size is 1
-    # for catalogues this small): RSD weighted 3PCF, equilateral 3PCF and 4PCF give the same counts,
-    # sums and statistics either way, and the 3PCF counts equal the oracle's
+@pytest.mark.parametrize("batch", ["7", "16"])
+@pytest.mark.parametrize("kmax1", ["", "64"])
+def test_apex_batches(eng, monkeypatch, batch, kmax1):
+    # apexes fetched from the work queue 7 or 16 at a time (the automatic size is 1 for catalogues
+    # this small), alone or with a 64-entry first-pass neighbour list, so that apexes in the middle
+    # of a batch go to the second pass: RSD weighted 3PCF, equilateral 3PCF and 4PCF give the
+    # counts, sums and statistics of single-apex fetches in one pass, and the 3PCF and 4PCF counts
+    # equal the oracle's
     rng = np.random.default_rng(17)
     d = C.clustered(4000, 100.0, 91)
     r = C.uniform(8000, 100.0, 92)
@@ -231,13 +234,22 @@ def test_apex_batches(eng, monkeypatch, batch):
     buf = (rng.random(len(xyz)) < 0.1).astype(np.int32)
     eng.set_catalogue(xyz, w, 4000, buf)
     monkeypatch.setenv("GS_APEX_BATCH", "1")
+    monkeypatch.setenv("GS_T3_KMAX", "")
+    monkeypatch.setenv("GS_T4_KMAX", "")
     base3 = eng.pcf3(2.0, 12.0, 4, nmu=3, precision="fp32")
     base_e = eng.pcf3_equi(2.0, 12.0, 4, precision="fp32")
     base4 = eng.pcf4(2.0, 8.0, 3)
     monkeypatch.setenv("GS_APEX_BATCH", batch)
+    monkeypatch.setenv("GS_T3_KMAX", kmax1)
+    monkeypatch.setenv("GS_T4_KMAX", kmax1)
     o3 = eng.pcf3(2.0, 12.0, 4, nmu=3, precision="fp32")
     oe = eng.pcf3_equi(2.0, 12.0, 4, precision="fp32")
     o4 = eng.pcf4(2.0, 8.0, 3)
+    if kmax1:
+        # mean neighbour counts here: ~87 within r < 12 (most 3PCF apexes overflow 64 entries),
+        # ~26 within r < 8 (the 4PCF's clustered apexes overflow)
+        ref4 = O.pcf4(xyz, w, 4000, O.Params(2.0, 8.0, 3, 1, False), buf=buf)
+        assert np.array_equal(o4["cnt"], ref4["cnt"]) and o4["stats"]["n_units"] == ref4["n_quads"]
     for o, b in ((o3, base3), (oe, base_e), (o4, base4)):
         assert np.array_equal(o["cnt"], b["cnt"])
         assert o["stats"]["n_units"] == b["stats"]["n_units"]
@@ -313,12 +325,64 @@ def test_4pcf_rejects_mu_bins_and_3pcf_large_neighbour_sets(eng):
     assert rc == L.GS_EINVAL
     rc = L.lib().gs_count_4pcf_tetra(eng._ctx, ctypes.byref(p), None, None, None, None, None, None)
     assert rc == L.GS_EINVAL
-    # every point within rmax of every other: more neighbours than the LDS list holds
-    dense = C.uniform(9000, 10.0, 2)
-    eng.set_catalogue(dense, None, len(dense))
-    with pytest.raises(GsError) as e:
-        eng.pcf3(0.1, 30.0, 3)
-    assert e.value.code == L.GS_ELIMIT
+    del GsError
+
+
+@pytest.mark.parametrize("prec,nmu,weighted,equi", [("fp32", 1, True, False), ("fp64", 1, False, False),
+                                                    ("fp32", 3, True, False), ("fp32", 1, True, True)])
+def test_3pcf_device_memory_neighbour_lists(eng, monkeypatch, prec, nmu, weighted, equi):
+    # every LDS list cut to 64 entries (about 90 neighbours per apex here): the apexes with more
+    # neighbours run in the last pass with their lists in device memory; counts equal the oracle's,
+    # and counts, sums and statistics equal those of the normal two-pass run
+    rng = np.random.default_rng(31)
+    d = C.clustered(3000, 90.0, 85)
+    r = C.uniform(6000, 90.0, 86)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(3000, 0.2, 87), np.ones(6000)]) if weighted else None
+    buf = (rng.random(len(xyz)) < 0.1).astype(np.int32)
+    p = O.Params(2.0, 12.0, 4, nmu, False)
+    eng.set_catalogue(xyz, w, 3000, buf)
+    run = (lambda: eng.pcf3_equi(p.rmin, p.rmax, p.nbins, nmu=nmu, precision=prec)) if equi else \
+        (lambda: eng.pcf3(p.rmin, p.rmax, p.nbins, nmu=nmu, precision=prec))
+    base = run()
+    monkeypatch.setenv("GS_T3_KMAX_LDS", "64")
+    out = run()
+    assert np.array_equal(out["cnt"], base["cnt"]) and np.array_equal(out["wsum"], base["wsum"])
+    for k in ("n_units", "n_binned", "n_candidates", "n_overflow"):
+        assert out["stats"][k] == base["stats"][k], k
+    ref = (O.pcf3_equi if equi else O.pcf3)(xyz, w, 3000, p, buf=buf)
+    assert np.array_equal(out["cnt"].reshape(ref["cnt"].shape), ref["cnt"])
+    assert _close(out["n2"].reshape(-1), ref["n2"].reshape(-1))
+
+
+def _esym(w, k):
+    """Elementary symmetric polynomial e_k of positive weights (k <= 3), from power sums."""
+    p1, p2, p3 = (np.sum(w ** j) for j in (1, 2, 3))
+    return [1.0, p1, (p1 * p1 - p2) / 2.0, (p1 ** 3 - 3.0 * p1 * p2 + 2.0 * p3) / 6.0][k]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_3pcf_more_than_4096_neighbours(eng, prec):
+    # 4300 points in a 10^3 box with rmax = 20 > its diagonal: every point is every other's
+    # neighbour (k = 4299, more than any LDS list holds).  With one bin, every triangle is counted
+    # once from each apex: class c (c randoms) holds 3 C(nd, 3-c) C(nr, c) triangles and the weight
+    # sum 3 e_{3-c}(w_data) e_c(w_rand)
+    from math import comb
+    nd, nr = 2000, 2300
+    xyz = C.uniform(nd + nr, 10.0, 41)
+    w = C.lognormal_weights(nd + nr, 0.2, 42)
+    eng.set_catalogue(xyz, w, nd)
+    out = eng.pcf3(1e-3, 20.0, 1, precision=prec)
+    cnt = out["cnt"].reshape(-1)
+    assert [int(x) for x in cnt] == [3 * comb(nd, 3 - c) * comb(nr, c) for c in range(4)]
+    ws = out["wsum"].reshape(-1)
+    for c in range(4):
+        exp = 3.0 * _esym(w[:nd], 3 - c) * _esym(w[nd:], c)
+        assert abs(ws[c] - exp) <= 1e-12 * exp, (c, ws[c], exp)
+    assert out["stats"]["n_units"] == (nd + nr) * comb(nd + nr - 1, 2)
+    # with mu bins, the RSD 3PCF summed over mu is the isotropic count
+    rsd = eng.pcf3(1e-3, 20.0, 1, nmu=4, precision=prec)
+    assert np.array_equal(rsd["cnt"].sum(axis=1).reshape(-1), cnt)
 
 
 def test_3pcf_rsd_exact_counts_vs_reference_runs(eng):
@@ -342,3 +406,52 @@ def test_3pcf_rsd_global_histogram(eng, weighted):
     p = O.Params(3.0, 15.0, 12, 10)
     print(f"case rsd global hist weighted={weighted}", flush=True)
     check3(eng, xyz, w, nd, p)
+
+
+@pytest.mark.parametrize("tetra,weighted", [(False, True), (False, False), (True, True)])
+def test_4pcf_device_memory_neighbour_lists(eng, monkeypatch, tetra, weighted):
+    # every LDS list cut to 64 entries: the apexes with more neighbours run in the last pass with
+    # their lists in device memory and the neighbour-neighbour bins evaluated on the fly (no link
+    # matrix); counts equal the oracle's and the normal run's, sums within 1e-12
+    rng = np.random.default_rng(33)
+    d = C.clustered(3000, 70.0, 91)
+    r = C.uniform(3000, 70.0, 92)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(3000, 0.2, 93), np.ones(3000)]) if weighted else None
+    buf = (rng.random(len(xyz)) < 0.1).astype(np.int32)
+    p = O.Params(3.0, 10.0, 3 if not tetra else 2, 1, False)
+    eng.set_catalogue(xyz, w, 3000, buf)
+    run = (lambda: eng.pcf4_tetra(p.rmin, p.rmax, p.nbins)) if tetra else (lambda: eng.pcf4(p.rmin, p.rmax, p.nbins))
+    base = run()
+    monkeypatch.setenv("GS_T4_KMAX_LDS", "64")
+    monkeypatch.setenv("GS_T4_KMAX", "64")
+    out = run()
+    assert np.array_equal(out["cnt"], base["cnt"])
+    for k in ("n_units", "n_binned", "n_candidates", "n_overflow"):
+        assert out["stats"][k] == base["stats"][k], k
+    assert _close(out["n2"], base["n2"]) and _close(out["disc_n"], base["disc_n"])
+    ref = (O.pcf4_tetra if tetra else O.pcf4)(xyz, w, 3000, p, buf=buf)
+    assert np.array_equal(out["cnt"], ref["cnt"])
+    assert _close(out["n2"], ref["n2"]) and _close(out["n3"], ref["n3"])
+    assert _close(out["disc_n"], ref["disc_n"]) and _close(out["disc_r"], ref["disc_r"])
+
+
+@pytest.mark.parametrize("tetra", [False, True])
+def test_4pcf_more_than_256_neighbours(eng, tetra):
+    # 300 points in a 10^3 box with rmax = 20 > its diagonal: k = 299 for every apex, more than an
+    # LDS list holds.  With one bin every quad is counted from each apex, and the random pattern
+    # follows from the id order (randoms after data, so among an apex's three neighbours in id
+    # order the randoms come last): pattern = r_i | (c >= 3) << 1 | (c >= 2) << 2 | (c >= 1) << 3
+    # for c randoms among them
+    from math import comb
+    nd, nr = 150, 150
+    xyz = C.uniform(nd + nr, 10.0, 51)
+    eng.set_catalogue(xyz, None, nd)
+    out = eng.pcf4_tetra(1e-3, 20.0, 1) if tetra else eng.pcf4(1e-3, 20.0, 1)
+    exp = np.zeros(16, np.int64)
+    for ri, n_i in ((0, nd), (1, nr)):
+        for c in range(4):
+            pat = ri | (int(c >= 3) << 1) | (int(c >= 2) << 2) | (int(c >= 1) << 3)
+            exp[pat] += n_i * comb(nd - (1 - ri), 3 - c) * comb(nr - ri, c)
+    assert np.array_equal(out["cnt"].reshape(-1, 16).sum(0).astype(np.int64), exp)
+    assert int(out["cnt"].sum()) == (nd + nr) * comb(nd + nr - 1, 3)

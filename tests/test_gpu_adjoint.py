@@ -235,3 +235,57 @@ def test_adjoint_linearity_and_extreme_scales(eng):
         err = np.max(np.abs(g - base)) / np.max(np.abs(base))
         print(f"scale {s:g}: {err:.3g}", flush=True)
         assert err < 1e-12, s
+
+
+def _empty_rr_bin_case():
+    # randoms on a lattice of spacing 2.5 have no pair closer than 2.5, so bin 0 = [0, 2) has
+    # RR = 0 (N3 = 0: xi infinite or undefined there) while the clustered data fill it
+    side = 60.0
+    d = C.clustered(1500, side, 71, members=5, sigma=1.5)
+    r = C.lattice(24, 2.5, 0.0)
+    w = np.concatenate([C.lognormal_weights(len(d), 0.2, 72), np.ones(len(r))])
+    return d, r, w, dict(rmin=0.0, rmax=20.0, nbins=10, nmu=4)
+
+
+def test_autograd_masked_empty_bin_gives_finite_gradient():
+    import torch
+    from gramsci_amd.autograd import weighted_xi
+    d, r, w0, p = _empty_rr_bin_case()
+    xyz_np = np.concatenate([d, r])
+    xyz = torch.from_numpy(xyz_np).cuda()
+    w = torch.from_numpy(w0).cuda().requires_grad_(True)
+    xi = weighted_xi(w, xyz, len(d), **p)
+    ok = torch.isfinite(xi)
+    assert not bool(ok[0].any()) and bool(ok[1:].all())
+    target = torch.full_like(xi, 0.5)
+    # mask the input of the loss (not its output, whose backward would form 0 * inf), so the
+    # masked bins receive an incoming gradient of exactly zero
+    xi_masked = torch.where(ok, xi, target)
+    loss = ((xi_masked - target) ** 2).sum()
+    loss.backward()
+    g = w.grad.detach().cpu().numpy()
+    assert np.isfinite(g).all()
+    # the same gradient from the oracle with the masked bin's dL/dN set to zero
+    P = O.Params(p["rmin"], p["rmax"], p["nbins"], p["nmu"], False)
+    ref = O.pcf2(xyz_np, w0, len(d), P)
+    x = ref["n2"] / np.where(ref["n3"] == 0, 1.0, ref["n3"])
+    gx = np.where(ref["n3"] != 0, 2.0 * (x - 0.5), 0.0)
+    g2 = np.where(ref["n3"] != 0, gx / np.where(ref["n3"] == 0, 1.0, ref["n3"]), 0.0)
+    g3 = np.where(ref["n3"] != 0, -gx * ref["n2"] / np.where(ref["n3"] == 0, 1.0, ref["n3"]) ** 2, 0.0)
+    g_ref = O.adjoint2(xyz_np, w0, len(d), P, g2, g3)
+    assert np.max(np.abs(g - g_ref)) <= 1e-10 * np.max(np.abs(g_ref))
+
+
+def test_autograd_unmasked_empty_bin_gives_nonfinite_gradient():
+    # a loss that depends on an infinite xi has no finite gradient: the wrapper passes the
+    # non-finite dL/dN on (no rounding to a finite number), and the adjoint propagates it
+    import torch
+    from gramsci_amd.autograd import weighted_xi
+    d, r, w0, p = _empty_rr_bin_case()
+    xyz = torch.from_numpy(np.concatenate([d, r])).cuda()
+    w = torch.from_numpy(w0).cuda().requires_grad_(True)
+    xi = weighted_xi(w, xyz, len(d), **p)
+    loss = ((xi - 0.5) ** 2).sum()
+    loss.backward()
+    g = w.grad.detach().cpu().numpy()
+    assert not np.isfinite(g).all()

@@ -124,6 +124,26 @@ def test_batched_3pcf_two_pass_overflow(eng, batch, monkeypatch):
         assert _close(out["n2"][r].reshape(-1), ref["n2"].reshape(-1))
 
 
+def test_batched_3pcf_device_memory_lists(eng, batch, monkeypatch):
+    # every LDS list cut to 64 entries: apexes with more neighbours (rmax = 12) are queued by the
+    # captured passes and counted after each replay with their lists in device memory; results
+    # equal the oracle's per region, and a replay gives the same counts and sums
+    xyz, buf, off, ndata = batch
+    w = C.lognormal_weights(len(xyz), 0.2, 31)
+    p = O.Params(2.0, 12.0, 3, 1, False)
+    monkeypatch.setenv("GS_T3_KMAX_LDS", "64")
+    monkeypatch.setenv("GS_T3_KMAX", "64")
+    eng.set_catalogue_batched(xyz, w, buf, off, ndata)        # fresh plans read the overrides
+    out = eng.pcf3_batched(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32", cell_div=2)
+    again = eng.pcf3_batched(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32", cell_div=2)
+    assert np.array_equal(out["cnt"], again["cnt"]) and np.array_equal(out["wsum"], again["wsum"])
+    for r in range(len(ndata)):
+        sl = slice(off[r], off[r + 1])
+        ref = O.pcf3(xyz[sl], w[sl], int(ndata[r]), p, buf=buf[sl])
+        assert np.array_equal(out["cnt"][r].reshape(ref["cnt"].shape), ref["cnt"]), r
+        assert _close(out["n2"][r].reshape(-1), ref["n2"].reshape(-1))
+
+
 def test_batch_survives_single_catalogue_uploads(eng, batch):
     # the batch keeps its own copy of the catalogue: uploading a single catalogue on the
     # same context (smaller or larger) changes neither replays of captured pipelines nor

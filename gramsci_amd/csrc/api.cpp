@@ -611,8 +611,8 @@ int build_grid(gs_ctx* c, double h)
     int rc;
     if ((rc = c->cell_of.reserve(n)) || (rc = c->cell_start.reserve(ncell + 1)) || (rc = c->fill.reserve(ncell + 1)) ||
         (rc = c->chunk_nc.reserve(ncell + 1)) || (rc = c->chunk_off.reserve(ncell + 1)) ||
-        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) || (rc = c->pl.reserve(n + kPad)) ||
-        (rc = c->flags.reserve(n + kPad)) || (rc = c->orig.reserve(n)))
+        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) || (rc = c->pl.reserve(n)) ||
+        (rc = c->flags.reserve(n)) || (rc = c->orig.reserve(n)))
         return rc;
     hipError_t e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
     if (e == hipSuccess) e = launch_cell_count(c->xyz.ptr, n, g, c->cell_of.ptr, c->fill.ptr, c->stream);

@@ -140,7 +140,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         (rc = B.nch.reserve(rows_tot + 1)) || (rc = B.row_chunk.reserve(rows_tot + 1)) ||
         (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(32)) ||
         (kind == 3 && (rc = B.ovf.reserve(2 * n + 3))) ||
-        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n + kPad)) || (rc = B.flags.reserve(n + kPad)) ||
+        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n)) || (rc = B.flags.reserve(n)) ||
         (rc = B.out.reserve(nout)))
         return rc;
     // per-plan parameter blob: regions, bins, stencil rows (kept for the plan's lifetime)

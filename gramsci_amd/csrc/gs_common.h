@@ -11,7 +11,6 @@ constexpr int kMaxBins = 127;   // int8 bin storage in the reference (node_modul
 constexpr int kWG = 256;        // work-group size of the counting kernels
 constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kChunk = 64;      // points per work item of the pair kernel (one per lane)
-constexpr int kPad = 4;         // spare entries after the sorted pl/flags arrays (four-entry scalar fetches)
 
 // Uniform cell grid over the catalogue's bounding box.  Points are stored
 // sorted by cell id = (cz*ny + cy)*nx + cx (x fastest), so one x-row of cells

@@ -8,36 +8,6 @@
 namespace gs {
 namespace dev {
 
-// Scalar loads: reads through a constant-address-space pointer at a wave-uniform
-// index become s_load instructions (into SGPRs).  Only for arrays that no kernel
-// writes while the reading kernel runs.
-typedef const __attribute__((address_space(4))) float* cfloat_p;
-typedef const __attribute__((address_space(4))) uint32_t* cu32_p;
-typedef const __attribute__((address_space(4))) double* cdouble_p;
-
-__device__ __forceinline__ float4 sload_f4(const float4* base, int k)
-{
-    const cfloat_p q = (cfloat_p)(const void*)base + 4 * (long)k;
-    return make_float4(q[0], q[1], q[2], q[3]);
-}
-
-__device__ __forceinline__ uint4 sload_u4(const uint32_t* base, int k)
-{
-    const cu32_p q = (cu32_p)(const void*)base + k;
-    return make_uint4(q[0], q[1], q[2], q[3]);
-}
-
-__device__ __forceinline__ double sload_w(const double4* base, int k)
-{
-    return ((cdouble_p)(const void*)base)[4 * (long)k + 3];
-}
-
-__device__ __forceinline__ double4 sload_d4(const double4* base, int k)
-{
-    const cdouble_p q = (cdouble_p)(const void*)base + 4 * (long)k;
-    return make_double4(q[0], q[1], q[2], q[3]);
-}
-
 __device__ __forceinline__ void glb_add128(unsigned long long* slot, unsigned long long lo, unsigned long long hi)
 {
     unsigned long long old = atomicAdd(slot, lo);

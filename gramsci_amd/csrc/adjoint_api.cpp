@@ -31,7 +31,7 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
     BinDesc B;
-    if ((rc = fill_bins(*p, (mst + 1) * h, ctx_xmax(c), v.wmax, B))) return rc;
+    if ((rc = fill_bins(*p, (mst + 1) * h, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
     if (!(v.wsum_d > 0) && v.n_data > 0) return fail(GS_EINVAL, "data weights sum to zero");
     if (!(v.wsum_r > 0) && v.n_rand > 0) return fail(GS_EINVAL, "random weights sum to zero");
     const double inv_wd = v.n_data > 0 ? (double)(1.0L / v.wsum_d) : 0.0;

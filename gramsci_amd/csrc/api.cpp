@@ -197,7 +197,7 @@ static float f_down(double x)
     return f;
 }
 
-int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B)
+int fill_bins(const gs_params& p, double rrel, double xmax, double wmax_d, double wmax_r, BinDesc& B)
 {
     memset(&B, 0, sizeof(B));
     int rc = gs_bin_thresholds(&p, B.t64);
@@ -234,18 +234,21 @@ int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc
     B.mu_ew = (float)(2.0 * std::sqrt(3.0) * ew * 4.0);
     B.mu_em = (float)(2.0 * std::sqrt(3.0) * em * 4.0);
     B.mu_c = (float)std::ldexp(1.0, -18);
-    B.fx_shift2 = fx_shift_pairs(wmax);
-    B.fx_shift3 = fx_shift_triples(wmax);
+    B.fx_sd = fx_shift_point(wmax_d);
+    B.fx_sr = fx_shift_point(wmax_r);
+    B.fx_shift3 = fx_shift_triples(std::max(wmax_d, wmax_r));
     return GS_OK;
 }
 
-// Fixed-point scales 2^F of the weight products: the largest pair product stays below 2^52
-// (the pair kernel rounds it to an integer with one FP64 addition of 2^52 and sums runs of up
-// to 33 in one 64-bit register), the largest triple product below 2^61.
-int fx_shift_pairs(double wmax)
+// Fixed-point scales 2^F of the weights and products.  Pairs: each weight is scaled by its
+// class's 2^F, below 2^26, so the largest pair product of every class stays below 2^52 (the
+// pair kernel rounds it to an integer with one FP64 addition of 2^52 and sums runs of up to
+// 33 in one 64-bit register) and each class sum is exact to 2^-52 of its own largest product.
+// Triples: the largest triple product stays below 2^61.
+int fx_shift_point(double wmax)
 {
     const double wm = std::max(wmax, 1e-300);
-    return std::min(1000, 52 - (int)std::ceil(std::log2(wm * wm * (1.0 + 1e-12))));
+    return std::min(500, 26 - (int)std::ceil(std::log2(wm * (1.0 + 1e-12))));
 }
 
 int fx_shift_triples(double wmax)
@@ -319,7 +322,7 @@ struct gs_ctx {
     int64_t n_data = 0, n_rand = 0;
     bool has_w = false, has_buf = false;
     long double wsum_d = 0, wsum_r = 0;
-    double wmax = 1.0;
+    double wmax = 1.0, wmax_d = 1.0, wmax_r = 1.0;
     double bbox[6] = {0, 0, 0, 0, 0, 0};
     DevBuf<double> xyz, w;
     DevBuf<int32_t> buf;
@@ -443,22 +446,26 @@ static int host_weight_stats(gs_ctx* c, const double* w)
 {
     c->wsum_d = 0;
     c->wsum_r = 0;
-    c->wmax = 1.0;
+    c->wmax = c->wmax_d = c->wmax_r = 1.0;
     const int64_t n = c->n_data + c->n_rand;
     if (!w) {
         c->wsum_d = (long double)c->n_data;
         c->wsum_r = (long double)c->n_rand;
         return GS_OK;
     }
-    double wm = 0.0;
+    double wm[2] = {0.0, 0.0};
     for (int64_t i = 0; i < n; ++i) {
         if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return fail(GS_EINVAL, "weights must be finite and >= 0 (w[%lld]=%g)", (long long)i, w[i]);
-        wm = std::max(wm, w[i]);
+        double& m = wm[i < c->n_data ? 0 : 1];
+        m = std::max(m, w[i]);
     }
-    const int k = wsum_scale(wm);
+    const double wall = std::max(wm[0], wm[1]);
+    const int k = wsum_scale(wall);
     c->wsum_d = fixed_sum(w, c->n_data, k);
     c->wsum_r = fixed_sum(w + c->n_data, c->n_rand, k);
-    c->wmax = wm > 0 ? wm : 1.0;
+    c->wmax = wall > 0 ? wall : 1.0;
+    c->wmax_d = wm[0] > 0 ? wm[0] : 1.0;
+    c->wmax_r = wm[1] > 0 ? wm[1] : 1.0;
     return GS_OK;
 }
 
@@ -472,10 +479,10 @@ static int device_weight_stats(gs_ctx* c)
     if ((rc = c->part.reserve(nblk * 6)) || (rc = c->wpart.reserve(nblk * 4 + 1))) return rc;
     unsigned long long* bad = c->wpart.ptr + nblk * 4;
     hipError_t e = launch_fill_u32(bad, 2, 0xffffffffu, c->stream);
-    if (e == hipSuccess) e = launch_wcheck(c->w.ptr, n, c->part.ptr, bad, nblk, c->stream);
-    std::vector<double> pmax(nblk);
+    if (e == hipSuccess) e = launch_wcheck(c->w.ptr, n, c->n_data, c->part.ptr, bad, nblk, c->stream);
+    std::vector<double> pmax(2 * nblk);
     unsigned long long first_bad = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(pmax.data(), c->part.ptr, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pmax.data(), c->part.ptr, 2 * nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(&first_bad, bad, sizeof(first_bad), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e, "weight check");
@@ -484,8 +491,10 @@ static int device_weight_stats(gs_ctx* c)
         (void)hipMemcpy(&v, c->w.ptr + first_bad, sizeof(double), hipMemcpyDeviceToHost);
         return fail(GS_EINVAL, "weights must be finite and >= 0 (w[%llu]=%g)", first_bad, v);
     }
-    double wm = 0.0;
-    for (double x : pmax) wm = std::max(wm, x);
+    double wmc[2] = {0.0, 0.0};
+    for (int b = 0; b < nblk; ++b)
+        for (int q = 0; q < 2; ++q) wmc[q] = std::max(wmc[q], pmax[2 * b + q]);
+    const double wm = std::max(wmc[0], wmc[1]);
     const int k = wsum_scale(wm);
     std::vector<unsigned long long> ps(nblk * 4);
     e = launch_wsum(c->w.ptr, n, c->n_data, k, c->wpart.ptr, nblk, c->stream);
@@ -500,6 +509,8 @@ static int device_weight_stats(gs_ctx* c)
     c->wsum_d = std::ldexp((long double)sd, -k);
     c->wsum_r = std::ldexp((long double)sr, -k);
     c->wmax = wm > 0 ? wm : 1.0;
+    c->wmax_d = wmc[0] > 0 ? wmc[0] : 1.0;
+    c->wmax_r = wmc[1] > 0 ? wmc[1] : 1.0;
     return GS_OK;
 }
 
@@ -680,7 +691,7 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     double xmax = 0;
     for (int k = 0; k < 6; ++k) xmax = std::max(xmax, std::fabs(c->bbox[k]));
     BinDesc B;
-    if ((rc = fill_bins(*p, (mst + 1) * h, xmax, c->wmax, B))) return rc;
+    if ((rc = fill_bins(*p, (mst + 1) * h, xmax, c->wmax_d, c->wmax_r, B))) return rc;
 
     // launch shape: work-group size and LDS histogram replicas (spreading same-bin
     // atomics).  256-thread work-groups (four waves per shared histogram) are kept
@@ -777,7 +788,8 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     }
     std::vector<long double> S(nbt);
     for (int k = 0; k < nbt; ++k)
-        S[k] = weighted ? fx_value(h_out[nbt + 2 * k], h_out[nbt + 2 * k + 1], B.fx_shift2) : (long double)h_out[k];
+        S[k] = weighted ? fx_value(h_out[nbt + 2 * k], h_out[nbt + 2 * k + 1], fx_pair_class_shift(B, k / (nb * nm)))
+                        : (long double)h_out[k];
     if (wsum)
         for (int k = 0; k < nbt; ++k) wsum[k] = (double)S[k];
     const long double Wd = c->wsum_d, Wr = c->wsum_r;
@@ -868,6 +880,8 @@ CtxView view(gs_ctx* c)
     v.wsum_d = c->wsum_d;
     v.wsum_r = c->wsum_r;
     v.wmax = c->wmax;
+    v.wmax_d = c->wmax_d;
+    v.wmax_r = c->wmax_r;
     for (int k = 0; k < 6; ++k) v.bbox[k] = c->bbox[k];
     v.g = c->g;
     v.pw = c->pw.ptr;

@@ -92,7 +92,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         d.pt_base = (int32_t)B.off[r];
         d.n_data = (int32_t)B.ndata[r];
         // the scale a launch over this region alone would use, so that sums match it bit for bit
-        d.fx_shift = kind == 3 ? fx_shift_triples(B.wmr[r]) : fx_shift_pairs(B.wmr[r]);
+        d.fx_shift = kind == 3 ? fx_shift_triples(B.wmr[r]) : fx_shift_point(B.wmr_d[r]);
+        d.fx_shift_rand = kind == 3 ? d.fx_shift : fx_shift_point(B.wmr_r[r]);
         row_base[r] = (int32_t)rows_tot;
         cells += nc;
         rows_tot += (int64_t)d.dim[1] * d.dim[2];
@@ -104,7 +105,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
     BinDesc bins;
     int rc;
-    if ((rc = fill_bins(p, kind == 3 ? 1.5 * p.rmax : (mst + 1) * h, B.xmax, B.wmax, bins))) return rc;
+    if ((rc = fill_bins(p, kind == 3 ? 1.5 * p.rmax : (mst + 1) * h, B.xmax, B.wmax_d, B.wmax_r, bins))) return rc;
 
     // triplet kernel: configuration and find_normal tables, LDS plan
     std::vector<uint16_t> tab3;
@@ -304,9 +305,15 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     P.graph = graph;
     P.exec = exec;
     P.nbt = nbt;
-    P.fx_shift = kind == 3 ? bins.fx_shift3 : bins.fx_shift2;
+    P.fx_shift = kind == 3 ? bins.fx_shift3 : bins.fx_sd;
     P.fx_shift_r.resize(nreg);
-    for (int r = 0; r < nreg; ++r) P.fx_shift_r[r] = reg[r].fx_shift;
+    P.fx_sd_r.resize(nreg);
+    P.fx_sr_r.resize(nreg);
+    for (int r = 0; r < nreg; ++r) {
+        P.fx_shift_r[r] = reg[r].fx_shift;
+        P.fx_sd_r[r] = reg[r].fx_shift;
+        P.fx_sr_r[r] = reg[r].fx_shift_rand;
+    }
     P.mst = mst;
     P.kind = kind;
     P.nh = nh;
@@ -367,15 +374,17 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
     B.wd.assign(nreg, 0.0L);
     B.wr.assign(nreg, 0.0L);
     B.wmr.assign(nreg, 1.0);
-    B.wmax = 1.0;
+    B.wmr_d.assign(nreg, 1.0);
+    B.wmr_r.assign(nreg, 1.0);
+    B.wmax = B.wmax_d = B.wmax_r = 1.0;
     B.xmax = 0.0;
-    double wm = 0.0;
+    double wm = 0.0, wmd = 0.0, wmrr = 0.0;
     for (int r = 0; r < nreg; ++r) {
         double* bb = &B.bbox[6 * r];
         for (int k = 0; k < 3; ++k) { bb[k] = 1e308; bb[3 + k] = -1e308; }
         if (off[r + 1] == off[r])
             for (int k = 0; k < 6; ++k) bb[k] = 0.0;
-        double wmr = 0.0;
+        double wmr = 0.0, wmr_c[2] = {0.0, 0.0};
         for (int64_t i = off[r]; i < off[r + 1]; ++i) {
             for (int k = 0; k < 3; ++k) {
                 const double x = xyz[3 * i + k];
@@ -388,10 +397,16 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
             if (!(wi >= 0.0) || !std::isfinite(wi)) return fail(GS_EINVAL, "weights must be finite and >= 0");
             wm = std::max(wm, wi);
             wmr = std::max(wmr, wi);
+            double& mc = wmr_c[i - off[r] < ndata[r] ? 0 : 1];
+            mc = std::max(mc, wi);
         }
+        wmd = std::max(wmd, wmr_c[0]);
+        wmrr = std::max(wmrr, wmr_c[1]);
         // the region's sums exactly as a single-catalogue upload of the region computes them
         if (w) {
             B.wmr[r] = wmr > 0 ? wmr : 1.0;
+            B.wmr_d[r] = wmr_c[0] > 0 ? wmr_c[0] : 1.0;
+            B.wmr_r[r] = wmr_c[1] > 0 ? wmr_c[1] : 1.0;
             const int k = wsum_scale(wmr);
             B.wd[r] = fixed_sum(w + off[r], ndata[r], k);
             B.wr[r] = fixed_sum(w + off[r] + ndata[r], off[r + 1] - off[r] - ndata[r], k);
@@ -400,7 +415,11 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
             B.wr[r] = (long double)(off[r + 1] - off[r] - ndata[r]);
         }
     }
-    if (w) B.wmax = wm > 0 ? wm : 1.0;
+    if (w) {
+        B.wmax = wm > 0 ? wm : 1.0;
+        B.wmax_d = wmd > 0 ? wmd : 1.0;
+        B.wmax_r = wmrr > 0 ? wmrr : 1.0;
+    }
     // the batch keeps its own copy of the catalogue, so that later single-catalogue
     // uploads on this context cannot change what the captured pipelines read
     B.has_w = w != nullptr;
@@ -468,7 +487,10 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         for (int k = 0; k < nbt; ++k) {
             if (cnt) cnt[(size_t)r * nbt + k] = hc[k];
             total += hc[k];
-            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], P->fx_shift_r[r]) : (long double)hc[k];
+            // class sums carry the region's scale of the class: DD 2 sd, DR sd + sr, RR 2 sr
+            const int cls = k / (nb * nm);
+            const int sh = cls == 0 ? 2 * P->fx_sd_r[r] : cls == 1 ? P->fx_sd_r[r] + P->fx_sr_r[r] : 2 * P->fx_sr_r[r];
+            S[k] = weighted ? fx_value(hw[2 * k], hw[2 * k + 1], sh) : (long double)hc[k];
             if (wsum) wsum[(size_t)r * nbt + k] = (double)S[k];
         }
         const long double Wd = B.wd[r], Wr = B.wr[r];

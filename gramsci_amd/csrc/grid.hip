@@ -35,23 +35,27 @@ __global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ xyz, in
 }
 
 // ---------------------------------------------------------------- weights
-__global__ __launch_bounds__(256) void k_wcheck(const double* __restrict__ w, int64_t n, double* __restrict__ part_max,
-                                                unsigned long long* __restrict__ bad)
+// per-block maxima of the data weights (i < n_data) and of the random weights: part_max[2 * block + class]
+__global__ __launch_bounds__(256) void k_wcheck(const double* __restrict__ w, int64_t n, int64_t n_data,
+                                                double* __restrict__ part_max, unsigned long long* __restrict__ bad)
 {
-    __shared__ double red[256];
-    double mx = 0.0;
+    __shared__ double red[2][256];
+    double mx[2] = {0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double v = w[i];
         if (!(v >= 0.0) || !isfinite(v)) atomicMin(bad, (unsigned long long)i);
-        else mx = fmax(mx, v);
+        else if (i < n_data) mx[0] = fmax(mx[0], v);
+        else mx[1] = fmax(mx[1], v);
     }
-    red[threadIdx.x] = mx;
+    red[0][threadIdx.x] = mx[0];
+    red[1][threadIdx.x] = mx[1];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        if (threadIdx.x < s)
+            for (int q = 0; q < 2; ++q) red[q][threadIdx.x] = fmax(red[q][threadIdx.x], red[q][threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) part_max[blockIdx.x] = red[0];
+    if (threadIdx.x < 2) part_max[2 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __device__ __forceinline__ void add128(unsigned long long& lo, unsigned long long& hi, unsigned long long l,
@@ -243,10 +247,10 @@ hipError_t launch_bbox(const double* xyz, int64_t n, double* part, int nblk, hip
     return hipGetLastError();
 }
 
-hipError_t launch_wcheck(const double* w, int64_t n, double* part_max, unsigned long long* bad, int nblk,
+hipError_t launch_wcheck(const double* w, int64_t n, int64_t n_data, double* part_max, unsigned long long* bad, int nblk,
                          hipStream_t st)
 {
-    hipLaunchKernelGGL(k_wcheck, dim3(nblk), dim3(256), 0, st, w, n, part_max, bad);
+    hipLaunchKernelGGL(k_wcheck, dim3(nblk), dim3(256), 0, st, w, n, n_data, part_max, bad);
     return hipGetLastError();
 }
 

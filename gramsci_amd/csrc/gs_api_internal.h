@@ -44,7 +44,8 @@ struct GsBatch {
     std::vector<double> bbox;            // [nreg][6]
     std::vector<long double> wd, wr;     // per-region weight sums
     std::vector<double> wmr;             // per-region largest weight (1 for unit weights)
-    double wmax = 1.0, xmax = 0.0;
+    std::vector<double> wmr_d, wmr_r;    // per-region largest data / random weight (1 for none)
+    double wmax = 1.0, wmax_d = 1.0, wmax_r = 1.0, xmax = 0.0;
     DevBuf<int32_t> d_off, d_row_base, cell_of, count, cell_start, nch, row_chunk, scan_tmp, queue, orig;
     DevBuf<int32_t> ovf;                 // 3PCF: queued apexes of the second and the device-memory pass
     DevBuf<unsigned char> big;           // 3PCF: device-memory neighbour lists
@@ -63,7 +64,8 @@ struct GsBatch {
         hipGraphExec_t exec = nullptr;
         int nbt = 0;
         int fx_shift = 0;
-        std::vector<int> fx_shift_r;     // per-region fixed-point scale (weighted runs)
+        std::vector<int> fx_shift_r;     // per-region fixed-point scale (weighted runs; triples)
+        std::vector<int> fx_sd_r, fx_sr_r;   // per-region pair scales of data and random weights
         int mst = 0;
         int kind = 2;                    // 2: pair kernel, 3: triplet kernel
         int nh = 0;                      // triplet histogram entries per region
@@ -88,7 +90,7 @@ struct CtxView {
     int64_t n_data, n_rand;
     bool has_w, has_buf;
     long double wsum_d, wsum_r;
-    double wmax;
+    double wmax, wmax_d, wmax_r;    // largest weight overall, of the data, of the randoms (1 if none)
     double bbox[6];
     GridDesc g;
     const double4* pw;
@@ -111,10 +113,13 @@ int ctx_upload(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf
 double round11(double h);
 int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
 int check_params(const gs_params* p, int mode);
-int fill_bins(const gs_params& p, double rrel, double xmax, double wmax, BinDesc& B);
-// fixed-point scales (exponents F of 2^F) of pair and triple weight products for a largest weight wmax
-int fx_shift_pairs(double wmax);
+int fill_bins(const gs_params& p, double rrel, double xmax, double wmax_d, double wmax_r, BinDesc& B);
+// fixed-point scales (exponents F of 2^F): of one point's weight in a pair product, for the largest
+// weight wmax of its class (wmax * 2^F < 2^26); of a triple product for the largest weight wmax
+int fx_shift_point(double wmax);
 int fx_shift_triples(double wmax);
+// exponent of a pair class sum: class 0 DD, 1 DR, 2 RR
+inline int fx_pair_class_shift(const BinDesc& B, int cls) { return cls == 0 ? 2 * B.fx_sd : cls == 1 ? B.fx_sd + B.fx_sr : 2 * B.fx_sr; }
 // sum of w[0..n) as the exact sum of the fixed-point integers wsum_fixed(w[i], k), as a long double
 long double fixed_sum(const double* w, int64_t n, int k);
 std::vector<Row> half_stencil(double h, double rmax, int m);

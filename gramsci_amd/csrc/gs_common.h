@@ -39,8 +39,12 @@ struct BinDesc {
     float   ua, ub;                // approximate bin: u = f(s)*ua + ub, f = sqrt (linear) or log2 (log)
     float   ms32;                  // mu_scale in FP32
     float   mu_ew, mu_em, mu_c;    // mu guard: dt = ms*(mu_ew/|sep| + mu_em/|mid| + mu_c)
-    // weighted fixed point: q = rint(w_i*w_j*... * 2^fx_shift)
-    int32_t fx_shift2, fx_shift3;
+    // weighted fixed point.  Pairs: each point's weight is scaled by 2^fx_sd (data) or 2^fx_sr
+    // (randoms), so that every scaled weight of its class is below 2^26, and a pair contributes
+    // q = rint(w_i' * w_j') < 2^52: the DD, DR and RR sums carry the scales 2^(2 fx_sd),
+    // 2^(fx_sd + fx_sr) and 2^(2 fx_sr), each set by its own class's largest weights.
+    // Triples: q = rint(w_i*w_j*w_k * 2^fx_shift3) with one scale.
+    int32_t fx_sd, fx_sr, fx_shift3;
 };
 
 // One independent region of a batched run (a domain-decomposed sub-volume
@@ -53,7 +57,10 @@ struct RegionDesc {
     int32_t row_base;        // first global x-row of the region
     int32_t pt_base;         // first point (input order and cell-sorted order)
     int32_t n_data;          // data points of the region (the rest are randoms)
-    int32_t fx_shift;        // the region's fixed-point scale of weight products (as its own launch)
+    // the region's fixed-point scales, as its own launch uses them: triples fx_shift; pairs fx_shift
+    // for data points and fx_shift_rand for randoms (BinDesc::fx_sd, fx_sr)
+    int32_t fx_shift;
+    int32_t fx_shift_rand;
 };
 
 // Exact, order-independent weight sums (the normalisations sum w over data

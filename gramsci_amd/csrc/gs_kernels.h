@@ -7,11 +7,11 @@ namespace gs {
 
 // grid.hip
 hipError_t launch_bbox(const double* xyz, int64_t n, double* part, int nblk, hipStream_t st);
-// weights: per-block maximum and first invalid (negative or non-finite) index (atomic min; *bad
-// starts at ~0), then per-block 128-bit fixed-point sums over data and randoms ([nblk][4]: data
-// lo, hi, randoms lo, hi)
-hipError_t launch_wcheck(const double* w, int64_t n, double* part_max, unsigned long long* bad, int nblk,
-                         hipStream_t st);
+// weights: per-block maxima of the data and of the random weights (part_max[2 * block + class]) and
+// the first invalid (negative or non-finite) index (atomic min; *bad starts at ~0), then per-block
+// 128-bit fixed-point sums over data and randoms ([nblk][4]: data lo, hi, randoms lo, hi)
+hipError_t launch_wcheck(const double* w, int64_t n, int64_t n_data, double* part_max, unsigned long long* bad,
+                         int nblk, hipStream_t st);
 hipError_t launch_wsum(const double* w, int64_t n, int64_t n_data, int k, unsigned long long* part, int nblk,
                        hipStream_t st);
 hipError_t launch_cell_count(const double* xyz, int64_t n, const GridDesc& g, int32_t* cell_of, int32_t* count,

@@ -210,7 +210,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     const float dt_ew = ms32 * mu_ew, dt_em = ms32 * mu_em;
     float dt_c = ms32 * mu_c, ubv = ub;
     if constexpr (W) asm volatile("" : "+v"(dt_c), "+v"(ubv));
-    int shift = B.fx_shift2;         // batched: the current region's scale
+    // fixed-point scales of data and random weights (batched: the current region's)
+    int sd = B.fx_sd, sr = B.fx_sr;
     const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
     const int n_chunks = batched ? *a.n_chunks_dev : a.n_chunks;
     const float h32 = a.g.h32;
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     // neighbours (adjacent points of one cell, data before randoms) often fall in the same
     // bin; their updates are merged in registers and reach the LDS histogram with one set of
     // atomics when the bin changes, which relieves same-address LDS atomic conflicts.
-    // The run's sum is one 64-bit integer: products are below 2^52 (fx_shift2) and a run ends
+    // The run's sum is one 64-bit integer: products are below 2^52 (fx_sd, fx_sr) and a run ends
     // after at most 33 increments, so it cannot overflow; the flush splits it at bit 32 into
     // the two LDS halves, which keeps value = hi * 2^32 + lo.  The update is written with
     // selects, not branches, so that no register copies are needed where paths merge.
@@ -267,7 +268,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
             nx = rd.dim[0]; ny = rd.dim[1]; nz = rd.dim[2];
             cb = rd.cell_base;
             row = grow - rd.row_base;
-            shift = rd.fx_shift;
+            sd = rd.fx_shift;
+            sr = rd.fx_shift_rand;
             if (R != cur_r) {
                 if constexpr (W) flush();
                 if (cur_r >= 0)
@@ -301,7 +303,9 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
         const int ri = fi & 1, ci = (fi & 2) ? 0 : 1;
         // a chunk of buffer points only (never apexes) gains nothing from a tile of buffer points only
         const bool chunk_buf = __ballot(ci != 0) == 0;
-        const double wi = Pi.w;
+        // the apex weight scaled by its class's 2^F (exact): a pair's product w_i' w_j' then carries
+        // the scale of its class (DD, DR or RR)
+        const double wis = ldexp(Pi.w, ri ? sr : sd);
         const float ax32 = (float)Pi.x, ay32 = (float)Pi.y, az32 = (float)Pi.z;   // absolute, for mu
         // histogram class offsets of this lane's pairs with a data / a random neighbour
         const int cb0 = ri ? nbm : 0, cb1 = cb0 + nbm;
@@ -330,7 +334,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                     const int cj = (fj & 2) ? 0 : 1;
                     const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
                     double x = 0.0;
-                    if constexpr (W) x = wi * wj;
+                    if constexpr (W) x = wis * wj;
                     accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mr == nm - 1 - m, ci, cj, x,
                                        a.out_cnt, a.out_w);
                 };
@@ -339,9 +343,9 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                     uint32_t fl = 2u;
                     if (lane < nt) {
                         const double4 Pj = a.pw[jt + lane];
-                        tx[lane] = Pj.x; ty[lane] = Pj.y; tz[lane] = Pj.z;
-                        if (W) tw[lane] = ldexp(Pj.w, shift);
                         fl = a.flags[jt + lane];
+                        tx[lane] = Pj.x; ty[lane] = Pj.y; tz[lane] = Pj.z;
+                        if (W) tw[lane] = ldexp(Pj.w, (fl & 1u) ? sr : sd);
                         tf[lane] = fl;
                     }
                     const bool skip = chunk_buf && __ballot((fl & 2u) == 0) == 0;
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         double ewj = 0.0;
                         if (!skip) {
                             e = a.pl[jl];
-                            if constexpr (W) ewj = ldexp(a.pw[jl].w, shift);
+                            if constexpr (W) ewj = ldexp(a.pw[jl].w, (ef & 1u) ? sr : sd);
                         }
                         // entry index and flags in one rotated register: ep = eidx << 2 | flags (-1: no
                         // entry), so the self-row test eidx >= jmin is ep >= jmin << 2
@@ -437,14 +441,14 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     const int cj = (ep & 2) ? 0 : 1;
                                     const int base = ((ep & 1) ? cb1 : cb0) + __umul24(b, nm);
                                     unsigned long long q = 0;
-                                    if constexpr (W) q = quantise_rn(wi * ewj);
+                                    if constexpr (W) q = quantise_rn(wis * ewj);
                                     if constexpr (W && !RSD) {
                                         if (ci + cj) put(base, (uint32_t)(ci + cj), q);
                                     } else if (W && mirror && mirrored) {
                                         put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
                                     } else {
                                         accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mirrored, ci, cj,
-                                                           wi * ewj, a.out_cnt, a.out_w);
+                                                           wis * ewj, a.out_cnt, a.out_w);
                                     }
                                 }
                             } while (false);

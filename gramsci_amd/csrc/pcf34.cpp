@@ -247,7 +247,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
     std::vector<Row> rows = full_stencil(h, p->rmax, mst);
     BinDesc B;
-    if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
+    if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
     std::vector<uint16_t> tab = equi ? std::vector<uint16_t>() : table3(nb);
     std::vector<uint8_t> ntab = nm > 1 ? normal_table(nm) : std::vector<uint8_t>();
 
@@ -452,7 +452,7 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
     std::vector<Row> rows = full_stencil(h, p->rmax, mst);
     BinDesc B;
-    if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax, B))) return rc;
+    if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
     std::vector<uint16_t> tab = tetra ? std::vector<uint16_t>() : table4(nb);
 
     // LDS plan of the second pass: the longest list (a power of two, 64..256, with its K x K link

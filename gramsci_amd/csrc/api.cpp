@@ -585,62 +585,82 @@ double round11(double h)
 }
 
 // Picks the cell edge: rmax / m, widened if the grid would exceed kMaxCells.
-static double choose_h(const gs_ctx* c, double rmax, int m)
+// Picks the cell edge for a catalogue with bounding box bbox: rmax / m rounded up to 11 significant
+// bits (then k*h is exact in FP32 for |k| < 2^13), widened while the grid would exceed max_cells.
+double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells)
 {
-    const int64_t kMaxCells = 1 << 24;
-    // round up to 11 significant bits: then k*h is exact in FP32 for |k| < 2^13
     int ex;
     double fr = std::frexp(rmax / m, &ex);
     double h = std::ldexp(std::ceil(std::ldexp(fr, 11)), ex - 11);
-    double ext = 0;
-    for (int k = 0; k < 3; ++k) ext = std::max(ext, c->bbox[3 + k] - c->bbox[k]);
     for (;;) {
         int64_t nc = 1;
-        for (int k = 0; k < 3; ++k) nc *= (int64_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
-        if (nc <= kMaxCells) break;
+        for (int k = 0; k < 3; ++k) nc *= (int64_t)std::floor((bbox[3 + k] - bbox[k]) / h) + 1;
+        if (nc <= max_cells) break;
         h = std::ldexp(std::ceil(std::ldexp(std::frexp(h * 1.25, &ex), 11)), ex - 11);
     }
-    (void)ext;
     return h;
+}
+
+static double choose_h(const gs_ctx* c, double rmax, int m)
+{
+    return choose_h_bbox(c->bbox, rmax, m, (int64_t)1 << 24);
+}
+
+// Grid of a catalogue (input-order xyz/w/buf in device memory) over its bounding box with cell edge
+// h, built into caller-provided device buffers; *nch receives the number of row chunks.
+int grid_build_raw(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
+                   const double* bbox, double h, const GridBufs& b, int64_t max_cells, GridDesc* g_out,
+                   int32_t* nch, hipStream_t st)
+{
+    GridDesc g{};
+    g.h = h;
+    int64_t ncell = 1;
+    for (int k = 0; k < 3; ++k) {
+        g.lo[k] = bbox[k];
+        g.dim[k] = (int32_t)std::floor((bbox[3 + k] - bbox[k]) / h) + 1;
+        ncell *= g.dim[k];
+    }
+    if (ncell > max_cells) return fail(GS_ELIMIT, "grid too large (%lld cells)", (long long)ncell);
+    g.ncell = (int32_t)ncell;
+    g.h32 = (float)h;
+    hipError_t e = launch_fill_u32(b.fill, ncell + 1, 0u, st);
+    if (e == hipSuccess) e = launch_cell_count(xyz, n, g, b.cell_of, b.fill, st);
+    if (e == hipSuccess) e = exclusive_scan(b.fill, b.cell_start, ncell + 1, b.scan_tmp, st);
+    if (e == hipSuccess) e = launch_fill_u32(b.fill, ncell + 1, 0u, st);
+    if (e == hipSuccess)
+        e = launch_scatter(xyz, w, buf, n, n_data, g, b.cell_of, b.cell_start, b.fill, b.pw, b.pl, b.flags, b.orig, st);
+    const int nrow = g.dim[1] * g.dim[2];
+    if (e == hipSuccess) e = launch_row_chunks(b.cell_start, g.dim[0], nrow, b.chunk_nc, st);
+    if (e == hipSuccess) e = exclusive_scan(b.chunk_nc, b.chunk_off, nrow + 1, b.scan_tmp, st);
+    int32_t nc = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nc, b.chunk_off + nrow, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "grid build");
+    *g_out = g;
+    *nch = nc;
+    return GS_OK;
 }
 
 int build_grid(gs_ctx* c, double h)
 {
     if (c->grid_valid && c->grid_h == h) return GS_OK;
     const int64_t n = c->n_data + c->n_rand;
-    GridDesc g{};
-    g.h = h;
     int64_t ncell = 1;
-    for (int k = 0; k < 3; ++k) {
-        g.lo[k] = c->bbox[k];
-        g.dim[k] = (int32_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
-        ncell *= g.dim[k];
-    }
+    for (int k = 0; k < 3; ++k) ncell *= (int64_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
     if (ncell > (1 << 26)) return fail(GS_ELIMIT, "grid too large (%lld cells)", (long long)ncell);
-    g.ncell = (int32_t)ncell;
-    g.h32 = (float)h;
     int rc;
     if ((rc = c->cell_of.reserve(n)) || (rc = c->cell_start.reserve(ncell + 1)) || (rc = c->fill.reserve(ncell + 1)) ||
         (rc = c->chunk_nc.reserve(ncell + 1)) || (rc = c->chunk_off.reserve(ncell + 1)) ||
         (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) || (rc = c->pl.reserve(n)) ||
         (rc = c->flags.reserve(n)) || (rc = c->orig.reserve(n)))
         return rc;
-    hipError_t e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
-    if (e == hipSuccess) e = launch_cell_count(c->xyz.ptr, n, g, c->cell_of.ptr, c->fill.ptr, c->stream);
-    if (e == hipSuccess) e = exclusive_scan(c->fill.ptr, c->cell_start.ptr, ncell + 1, c->scan_tmp.ptr, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->fill.ptr, 0, (ncell + 1) * sizeof(int32_t), c->stream);
-    if (e == hipSuccess)
-        e = launch_scatter(c->xyz.ptr, c->has_w ? c->w.ptr : nullptr, c->has_buf ? c->buf.ptr : nullptr, n, c->n_data,
-                           g, c->cell_of.ptr, c->cell_start.ptr, c->fill.ptr, c->pw.ptr, c->pl.ptr, c->flags.ptr,
-                           c->orig.ptr, c->stream);
-    const int nrow = g.dim[1] * g.dim[2];
-    if (e == hipSuccess) e = launch_row_chunks(c->cell_start.ptr, g.dim[0], nrow, c->chunk_nc.ptr, c->stream);
-    if (e == hipSuccess) e = exclusive_scan(c->chunk_nc.ptr, c->chunk_off.ptr, nrow + 1, c->scan_tmp.ptr, c->stream);
+    GridBufs b{c->cell_of.ptr, c->cell_start.ptr, c->fill.ptr, c->chunk_nc.ptr, c->chunk_off.ptr, c->scan_tmp.ptr,
+               c->pw.ptr, c->pl.ptr, c->flags.ptr, c->orig.ptr};
+    GridDesc g;
     int32_t nch = 0;
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(&nch, c->chunk_off.ptr + nrow, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return hip_fail(e, "grid build");
+    if ((rc = grid_build_raw(c->xyz.ptr, c->has_w ? c->w.ptr : nullptr, c->has_buf ? c->buf.ptr : nullptr, n,
+                             c->n_data, c->bbox, h, b, (int64_t)1 << 26, &g, &nch, c->stream)))
+        return rc;
     c->g = g;
     c->grid_h = h;
     c->n_chunks = nch;
@@ -648,14 +668,13 @@ int build_grid(gs_ctx* c, double h)
     return GS_OK;
 }
 
-int default_cell_div(const gs_ctx* c, double rmax, int mode)
+// Cells per rmax of the pair kernel for n points in bounding box bbox: the finest grid (m <= 8)
+// whose cells still hold ~32 points or more.
+int pair_cell_div(int64_t n, const double* bbox, double rmax)
 {
-    // aim at ~200 points per cell for the pair kernel (one 256-lane chunk per cell)
-    const int64_t n = c->n_data + c->n_rand;
     double vol = 1.0;
-    for (int k = 0; k < 3; ++k) vol *= std::max(c->bbox[3 + k] - c->bbox[k], rmax * 1e-3);
+    for (int k = 0; k < 3; ++k) vol *= std::max(bbox[3 + k] - bbox[k], rmax * 1e-3);
     const double dens = n / vol;
-    if (mode != 2) return 1;
     int best = 1;
     for (int m = 1; m <= 8; ++m) {
         const double h = rmax / m;
@@ -664,9 +683,103 @@ int default_cell_div(const gs_ctx* c, double rmax, int mode)
     return best;
 }
 
+int default_cell_div(const gs_ctx* c, double rmax, int mode)
+{
+    if (mode != 2) return 1;
+    return pair_cell_div(c->n_data + c->n_rand, c->bbox, rmax);
+}
+
 }  // namespace gs
 
 // ====================================================================== 2PCF
+
+namespace gs {
+
+// Launch shape of the pair kernel: work-group size and LDS histogram replicas (spreading same-bin
+// atomics).  256-thread work-groups (four waves per shared histogram) are kept when they reach 16
+// resident waves per CU; otherwise larger work-groups share one histogram among more waves to raise
+// residency.  Among shapes with the same residency, more replicas win.  Cached per variant.
+int pair2_shape(int nb, int nm, bool fp64, bool weighted, bool logb, int* reps_out, int* wg_out, int* waves_out)
+{
+    const bool rsd = nm >= 2;
+    int reps = 1, wg = kWG, best_waves = 0;
+    const char* ev_reps = getenv("GS_PAIR_REPS");    // tuning overrides (unset or empty: automatic)
+    const char* ev_wg = getenv("GS_PAIR_WG");
+    if (ev_reps && !*ev_reps) ev_reps = nullptr;
+    if (ev_wg && !*ev_wg) ev_wg = nullptr;
+    char key[96];
+    snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%s/%s", nb, nm, (int)fp64, (int)rsd, (int)weighted, (int)logb,
+             ev_reps ? ev_reps : "", ev_wg ? ev_wg : "");
+    static std::mutex mu;
+    static std::map<std::string, std::array<int, 3>> shapes;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = shapes.find(key);
+    if (it != shapes.end()) {
+        reps = it->second[0]; wg = it->second[1]; best_waves = it->second[2];
+    } else {
+        for (int pass = 0; pass < 2 && (pass == 0 || best_waves < 16); ++pass) {
+            for (int r = 1; r <= 32; r *= 2) {
+                for (int w = pass == 0 ? 256 : 384; w <= (pass == 0 ? 256 : 1024); w += 128) {
+                    if (ev_reps && atoi(ev_reps) != r) continue;
+                    if (ev_wg && atoi(ev_wg) != w) continue;
+                    if (pair2_lds_bytes(nb, rsd ? nm : 1, r, fp64, weighted, false, w) > 160 * 1024) continue;
+                    const int waves = pair2_occupancy(nb, nm, r, fp64, rsd, weighted, logb, false, w) * (w / kWave);
+                    if (waves > best_waves || (waves == best_waves && r > reps && w == wg)) {
+                        best_waves = waves;
+                        reps = r;
+                        wg = w;
+                    }
+                }
+            }
+        }
+        shapes[key] = {reps, wg, best_waves};
+    }
+    if (best_waves == 0) return fail(GS_ELIMIT, "histogram of %d bins does not fit in LDS", 3 * nb * nm);
+    *reps_out = reps;
+    *wg_out = wg;
+    *waves_out = best_waves;
+    return GS_OK;
+}
+
+// Pair kernel outputs (counts [3][nb][nm], 128-bit class sums, statistics) -> the C ABI's outputs:
+// cnt, the class sums of raw-weight products, N2 = DD/Wd^2 - DR/(Wd Wr) + RR/Wr^2 and N3 = RR/Wr^2
+// (gramsci.F90:373-379 over ordered pairs), and the pair statistics of st (times left to the caller).
+void combine2(const unsigned long long* h_out, int nb, int nm, const BinDesc& B, bool weighted, long double Wd,
+              long double Wr, uint64_t* cnt, double* wsum, double* n2, double* n3, gs_stats* st)
+{
+    const int nbt = 3 * nb * nm;
+    uint64_t total = 0;
+    for (int k = 0; k < nbt; ++k) {
+        if (cnt) cnt[k] = h_out[k];
+        total += h_out[k];
+    }
+    std::vector<long double> S(nbt);
+    for (int k = 0; k < nbt; ++k)
+        S[k] = weighted ? fx_value(h_out[nbt + 2 * k], h_out[nbt + 2 * k + 1], fx_pair_class_shift(B, k / (nb * nm)))
+                        : (long double)h_out[k];
+    if (wsum)
+        for (int k = 0; k < nbt; ++k) wsum[k] = (double)S[k];
+    for (int b = 0; b < nb * nm; ++b) {
+        const long double dd = S[b], dr = S[nb * nm + b], rr = S[2 * nb * nm + b];
+        long double v2 = 0, v3 = 0;
+        if (Wd > 0) v2 += dd / (Wd * Wd);
+        if (Wd > 0 && Wr > 0) v2 -= dr / (Wd * Wr);
+        if (Wr > 0) { v2 += rr / (Wr * Wr); v3 = rr / (Wr * Wr); }
+        if (n2) n2[b] = (double)v2;
+        if (n3) n3[b] = (double)v3;
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->n_units = total;
+        st->n_binned = total;
+        st->n_candidates = h_out[3 * nbt + 0];
+        st->n_recheck = h_out[3 * nbt + 1];
+        st->n_overflow = h_out[3 * nbt + 2];
+    }
+}
+
+}  // namespace gs
+
 
 extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2, double* n3,
                              gs_stats* st)
@@ -693,47 +806,8 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     BinDesc B;
     if ((rc = fill_bins(*p, (mst + 1) * h, xmax, c->wmax_d, c->wmax_r, B))) return rc;
 
-    // launch shape: work-group size and LDS histogram replicas (spreading same-bin
-    // atomics).  256-thread work-groups (four waves per shared histogram) are kept
-    // when they reach 16 resident waves per CU; otherwise larger work-groups share
-    // one histogram among more waves to raise residency.  Among shapes with the
-    // same residency, more replicas win.
     int reps = 1, wg = kWG, best_waves = 0;
-    {
-        const char* ev_reps = getenv("GS_PAIR_REPS");    // tuning overrides (unset or empty: automatic)
-        const char* ev_wg = getenv("GS_PAIR_WG");
-        if (ev_reps && !*ev_reps) ev_reps = nullptr;
-        if (ev_wg && !*ev_wg) ev_wg = nullptr;
-        char key[96];
-        snprintf(key, sizeof(key), "%d/%d/%d/%d/%d/%d/%s/%s", nb, nm, (int)fp64, (int)rsd, (int)weighted,
-                 (int)(p->log_bins != 0), ev_reps ? ev_reps : "", ev_wg ? ev_wg : "");
-        static std::mutex mu;
-        static std::map<std::string, std::array<int, 3>> shapes;
-        std::lock_guard<std::mutex> lock(mu);
-        auto it = shapes.find(key);
-        if (it != shapes.end()) {
-            reps = it->second[0]; wg = it->second[1]; best_waves = it->second[2];
-        } else {
-            for (int pass = 0; pass < 2 && (pass == 0 || best_waves < 16); ++pass) {
-                for (int r = 1; r <= 32; r *= 2) {
-                    for (int w = pass == 0 ? 256 : 384; w <= (pass == 0 ? 256 : 1024); w += 128) {
-                        if (ev_reps && atoi(ev_reps) != r) continue;
-                        if (ev_wg && atoi(ev_wg) != w) continue;
-                        if (pair2_lds_bytes(nb, rsd ? nm : 1, r, fp64, weighted, false, w) > 160 * 1024) continue;
-                        const int waves =
-                            pair2_occupancy(nb, nm, r, fp64, rsd, weighted, p->log_bins != 0, false, w) * (w / kWave);
-                        if (waves > best_waves || (waves == best_waves && r > reps && w == wg)) {
-                            best_waves = waves;
-                            reps = r;
-                            wg = w;
-                        }
-                    }
-                }
-            }
-            shapes[key] = {reps, wg, best_waves};
-        }
-    }
-    if (best_waves == 0) return fail(GS_ELIMIT, "histogram of %d bins does not fit in LDS", nbt);
+    if ((rc = pair2_shape(nb, nm, fp64, weighted, p->log_bins != 0, &reps, &wg, &best_waves))) return rc;
 
     const size_t nout = (size_t)nbt + 2 * (size_t)nbt + 4;
     if ((rc = c->params.reserve(sizeof(BinDesc) + rows.size() * sizeof(Row) + 64)) || (rc = c->out.reserve(nout)) ||
@@ -780,41 +854,14 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(e, "pair kernel");
 
-    // outputs: counts [3][nb][nm]; the kernel's layout is (cls*nb + b)*nm + m, identical
-    uint64_t total = 0;
-    for (int k = 0; k < nbt; ++k) {
-        if (cnt) cnt[k] = h_out[k];
-        total += h_out[k];
-    }
-    std::vector<long double> S(nbt);
-    for (int k = 0; k < nbt; ++k)
-        S[k] = weighted ? fx_value(h_out[nbt + 2 * k], h_out[nbt + 2 * k + 1], fx_pair_class_shift(B, k / (nb * nm)))
-                        : (long double)h_out[k];
-    if (wsum)
-        for (int k = 0; k < nbt; ++k) wsum[k] = (double)S[k];
-    const long double Wd = c->wsum_d, Wr = c->wsum_r;
-    for (int b = 0; b < nb * nm; ++b) {
-        const long double dd = S[b], dr = S[nb * nm + b], rr = S[2 * nb * nm + b];
-        long double v2 = 0, v3 = 0;
-        if (Wd > 0) v2 += dd / (Wd * Wd);
-        if (Wd > 0 && Wr > 0) v2 -= dr / (Wd * Wr);
-        if (Wr > 0) { v2 += rr / (Wr * Wr); v3 = rr / (Wr * Wr); }
-        if (n2) n2[b] = (double)v2;
-        if (n3) n3[b] = (double)v3;
-    }
+    combine2(h_out.data(), nb, nm, B, weighted, c->wsum_d, c->wsum_r, cnt, wsum, n2, n3, st);
     if (st) {
-        memset(st, 0, sizeof(*st));
         float ms_grid = 0, ms_count = 0;
         (void)hipEventElapsedTime(&ms_grid, c->ev[0], c->ev[1]);
         (void)hipEventElapsedTime(&ms_count, c->ev[2], c->ev[3]);
         st->ms_grid = ms_grid;
         st->ms_count = ms_count;
         st->ms_kernel_avg = ms_count;
-        st->n_units = total;
-        st->n_binned = total;
-        st->n_candidates = h_out[3 * nbt + 0];
-        st->n_recheck = h_out[3 * nbt + 1];
-        st->n_overflow = h_out[3 * nbt + 2];
         st->cell_div = mst;
     }
     return GS_OK;

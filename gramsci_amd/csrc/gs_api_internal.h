@@ -112,6 +112,23 @@ int ctx_set_device(gs_ctx* c);
 int ctx_upload(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf, int64_t n);
 double round11(double h);
 int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
+// device buffers of a grid build (sizes: cell_of, pw, pl, flags, orig n; cell_start, fill, chunk_nc,
+// chunk_off ncell + 1; scan_tmp scan_scratch_size(ncell + 1))
+struct GridBufs {
+    int32_t *cell_of, *cell_start, *fill, *chunk_nc, *chunk_off, *scan_tmp;
+    double4* pw;
+    float4* pl;
+    uint32_t* flags;
+    int32_t* orig;
+};
+int grid_build_raw(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
+                   const double* bbox, double h, const GridBufs& b, int64_t max_cells, GridDesc* g_out,
+                   int32_t* nch, hipStream_t st);
+int pair2_shape(int nb, int nm, bool fp64, bool weighted, bool logb, int* reps, int* wg, int* waves);
+double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells);
+int pair_cell_div(int64_t n, const double* bbox, double rmax);
+void combine2(const unsigned long long* h_out, int nb, int nm, const BinDesc& B, bool weighted, long double Wd,
+              long double Wr, uint64_t* cnt, double* wsum, double* n2, double* n3, gs_stats* st);
 int check_params(const gs_params* p, int mode);
 int fill_bins(const gs_params& p, double rrel, double xmax, double wmax_d, double wmax_r, BinDesc& B);
 // fixed-point scales (exponents F of 2^F): of one point's weight in a pair product, for the largest

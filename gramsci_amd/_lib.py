@@ -93,6 +93,7 @@ def lib():
     L.gs_count_2pcf_batched.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
     L.gs_count_3pcf_batched.argtypes = [vp, pp, u64p, dp, dp, dp, sp]
     L.gs_adjoint_2pcf_device.argtypes = [vp, pp, dp, dp, vp, sp]
+    L.gs_count_2pcf_noalloc.argtypes = [vp, pp, vp, vp, vp, ctypes.c_int64, ctypes.c_int64, u64p, dp, dp, dp, sp]
     for name in exported_names():
         getattr(L, name)  # every declared symbol must resolve
         if name not in ("gs_last_error", "gs_version"):

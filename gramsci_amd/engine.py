@@ -187,6 +187,30 @@ class Engine:
                                         L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
         return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, xi=_xi(n2, n3), stats=st.as_dict())
 
+    def pcf2_noalloc(self, xyz, w, n_data, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32",
+                     cell_div=0, buf=None):
+        """gs_count_2pcf_noalloc: the 2PCF of a catalogue in device memory (torch CUDA tensors:
+        xyz (n, 3) float64, w (n,) float64 or None, buf (n,) int32 or None) with no device
+        allocation; the context's own catalogue is not touched."""
+        import torch
+        xyz = xyz.to(torch.float64).contiguous()
+        w = None if w is None else w.to(torch.float64).contiguous()
+        buf = None if buf is None else buf.to(torch.int32).contiguous()
+        n = int(xyz.shape[0])
+        p = self.params(rmin, rmax, nbins, nmu, log_bins, precision, cell_div)
+        cnt = np.zeros((3, nbins, nmu), np.uint64)
+        wsum = np.zeros((3, nbins, nmu))
+        n2 = np.zeros((nbins, nmu))
+        n3 = np.zeros((nbins, nmu))
+        st = L.gs_stats()
+        torch.cuda.current_stream().synchronize()
+        L.check(self._lib.gs_count_2pcf_noalloc(
+            self._ctx, ctypes.byref(p), ctypes.c_void_p(xyz.data_ptr()),
+            None if w is None else ctypes.c_void_p(w.data_ptr()),
+            None if buf is None else ctypes.c_void_p(buf.data_ptr()), int(n_data), n - int(n_data),
+            L.ptr(cnt, ctypes.c_uint64), L.ptr(wsum), L.ptr(n2), L.ptr(n3), ctypes.byref(st)))
+        return dict(cnt=cnt, wsum=wsum, n2=n2, n3=n3, xi=_xi(n2, n3), stats=st.as_dict())
+
     def adjoint2(self, dl_dn2, dl_dn3, rmin, rmax, nbins, nmu=1, log_bins=False, precision="fp32", cell_div=0,
                  out=None):
         """dL/dw for the raw weights of the current catalogue, given dL/dN2 and

@@ -104,6 +104,21 @@ int gs_set_catalogue_device(gs_ctx* ctx, const double* d_xyz, const double* d_w,
 int gs_count_2pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
                   double* n2, double* n3, gs_stats* st);
 
+/* gs_count_2pcf for a catalogue in device memory (d_xyz [n][3], d_w [n] or
+ * NULL, d_buf [n] or NULL; data first, then randoms), allocating nothing: the
+ * sorted catalogue, grid, parameters, accumulators and counters live in one
+ * file-scope device array of the library (about 50 MB per GPU, held until the
+ * process ends).  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Limits: at most 524,288 points
+ * (GS_ELIMIT beyond) and 1,048,576 grid cells (the cell edge is widened to
+ * fit).  Outputs as gs_count_2pcf, in host memory.  The catalogue uploaded to
+ * ctx (if any) is neither used nor changed.  Calls that overlap on one GPU
+ * race for its array. */
+int gs_count_2pcf_noalloc(gs_ctx* ctx, const gs_params* p, const double* d_xyz, const double* d_w,
+                          const int32_t* d_buf, int64_t n_data, int64_t n_rand, uint64_t* cnt, double* wsum,
+                          double* n2, double* n3, gs_stats* st);
+
 /* 3PCF, all configurations: create_graph + query_graph_3pcf_all (gramsci.F90:463-545).
  * ncfg = C(nbins+2,3), in the reference's i<=j<=k order.  nmu >= 2 is the
  * reference's RSD 3PCF: each triplet also goes to the mu bin

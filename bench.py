@@ -214,7 +214,8 @@ def bench_rmu(args, legs=True):
         out, t_step, t_kern, t_grid = time_gpu(eng, dxyz, dw, nd, prec, args.steps, args.warmup)
         res[prec] = dict(out=out, t_step=t_step, t_kern=t_kern, t_grid=t_grid)
         print(f"[bench] {prec}: {t_step * 1e3:.1f} ms/step, pair kernel {t_kern:.1f} ms, grid {t_grid:.2f} ms, "
-              f"binned {out['stats']['n_binned']:.4g}, rechecks {out['stats']['n_recheck']}", file=sys.stderr,
+              f"binned {out['stats']['n_binned']:.4g}, candidates {out['stats']['n_candidates']:.4g}, "
+              f"rechecks {out['stats']['n_recheck']}", file=sys.stderr,
               flush=True)
     if "fp32" in res and "fp64" in res:
         assert np.array_equal(res["fp32"]["out"]["cnt"], res["fp64"]["out"]["cnt"]), "fp32/fp64 counts differ"
@@ -229,6 +230,7 @@ def bench_rmu(args, legs=True):
                      "2PCF (r,mu): 1M NS-clustered galaxies (lognormal w) + 3M uniform randoms, 1000^3 box, "
                      "1<r<150, 30 log bins, nmu=20",
                      n_points=int(len(xyz)), binned_pairs_per_step=int(U),
+                     candidate_pairs_per_step=int(R["out"]["stats"]["n_candidates"]),
                      precision="FP32 + FP64 guard-band recheck" if main_prec == "fp32" else "plain FP64")
     line["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                         "frac": achieved / peak, "traffic": None,

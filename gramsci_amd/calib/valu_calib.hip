@@ -53,15 +53,17 @@ int gs_calib_fma32(int blocks, int iters, float* out, int64_t out_floats, float*
     hipError_t e = hipEventCreate(&e0);
     if (e != hipSuccess) return (int)e;
     e = hipEventCreate(&e1);
-    if (e != hipSuccess) { hipEventDestroy(e0); return (int)e; }
-    hipEventRecord(e0, 0);
-    hipLaunchKernelGGL(k_valu_fma32, dim3(blocks), dim3(kThreads), 0, 0, out, iters, 0.999999f, 1e-7f);
-    e = hipGetLastError();
-    hipEventRecord(e1, 0);
+    if (e != hipSuccess) { (void)hipEventDestroy(e0); return (int)e; }
+    e = hipEventRecord(e0, 0);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_valu_fma32, dim3(blocks), dim3(kThreads), 0, 0, out, iters, 0.999999f, 1e-7f);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return (int)e;
 }
 

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
     const float e_lo = nm == 127 ? 0.0f : 1.0f, e_hi = nm == 127 ? 127.0f : nmm1f;
     const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
     const int nrow = ny * nz;
-    const float h32 = a.g.h32;
+    const float h32 = a.g.h32, hx32 = a.g.hx32;    // cell edges in y/z and in x
     const double inv_wd = a.inv_wd, inv_wr = a.inv_wr;
 
     unsigned long long cand = 0, rechk = 0;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
                         if constexpr (RSD) exact_mu(Pi.x, Pi.y, Pi.z, jx, jy, jz, ms, nm, m, mr);
                     } else {
                         const float4 tj = tl[k];
-                        const float dx = fmaf(tj.w - Li.w, h32, tj.x - Li.x);
+                        const float dx = fmaf(tj.w - Li.w, hx32, tj.x - Li.x);
                         const float dy = tj.y - yi_r, dz = tj.z - zi_r;
                         const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         if (!(s > rej_lo && s < rej_hi)) continue;
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2h(AdjArgs a)
     const float e_lo = nm == 127 ? 0.0f : 1.0f, e_hi = nm == 127 ? 127.0f : nmm1f;
     const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
     const int nrow = ny * nz;
-    const float h32 = a.g.h32;
+    const float h32 = a.g.h32, hx32 = a.g.hx32;    // cell edges in y/z and in x
     // normalised weights times w_scale: w / Wd for data, -w / Wr for randoms
     const double wsd = a.inv_wd * a.w_scale, wsr = -a.inv_wr * a.w_scale;
 
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2h(AdjArgs a)
                 long long qi = 0, qj = 0;
                 for (int k = 0; k < (skip ? 0 : P); ++k) {
                     if (vi && ep >= jmin4) do {
-                        const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
+                        const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
                         const float dy = e.y - yi_r, dz = e.z - zi_r;
                         const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                         if (!(s > rej_lo && s < rej_hi)) break;

@@ -23,10 +23,11 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
     CtxView v = view(c);
     const int64_t n = v.n_data + v.n_rand;
     const int m = p->cell_div > 0 ? p->cell_div : default_cell_div(c, p->rmax, 2);
+    const int sx = pair_cell_sx();
 
     (void)hipEventRecord(v.ev[0], v.stream);
     double h;
-    if ((rc = ctx_build_grid(c, p->rmax, m, &h))) return rc;
+    if ((rc = ctx_build_grid(c, p->rmax, m, sx, &h))) return rc;
     (void)hipEventRecord(v.ev[1], v.stream);
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
@@ -59,7 +60,7 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
                 half = false;
         }
     }
-    std::vector<Row> rows = half ? half_stencil(h, p->rmax, mst) : full_stencil(h, p->rmax, mst);
+    std::vector<Row> rows = half ? half_stencil(h, h / sx, p->rmax, mst) : full_stencil(h, h / sx, p->rmax, mst);
 
     GsExtra& X = *v.extra;
     const size_t g_off = ((sizeof(BinDesc) + 15) / 16) * 16;

@@ -257,45 +257,50 @@ int fx_shift_triples(double wmax)
     return std::min(1000, 61 - (int)std::ceil(std::log2(wm * wm * wm * (1.0 + 1e-12))));
 }
 
-// Half stencil of x-rows for cells of edge h and a search radius rmax, m = ceil(rmax/h).
-std::vector<Row> half_stencil(double h, double rmax, int m)
+// Half stencil of x-rows for cells of edge h in y/z and hx in x, search radius rmax, m = ceil(rmax/h).
+// Row (dy, dz) takes x-offsets dx with gap(dy)^2 h^2 + gap(dz)^2 h^2 + gap(dx)^2 hx^2 <= rmax^2 (plus a
+// relative margin of 1e-6), gap(d) = max(|d| - 1, 0) being the fewest whole cells between two cells d
+// apart: no pair within rmax is left out, and a fine hx cuts each row close to the sphere.
+std::vector<Row> half_stencil(double h, double hx, double rmax, int m)
 {
     std::vector<Row> rows;
     auto gap = [](int d) { return (double)std::max(std::abs(d) - 1, 0); };
-    const double lim = (rmax / h) * (1.0 + 1e-6) + 1e-9;
+    const double lim = rmax * (1.0 + 1e-6) + 1e-9 * h;
+    const double lim2 = lim * lim;
+    const int mxmax = (int)std::ceil(lim / hx) + 1;
     for (int dz = 0; dz <= m; ++dz)
         for (int dy = -m; dy <= m; ++dy) {
             if (dz == 0 && dy < 0) continue;
-            const double gyz = gap(dy) * gap(dy) + gap(dz) * gap(dz);
-            if (gyz > lim * lim) continue;
+            const double gyz = (gap(dy) * gap(dy) + gap(dz) * gap(dz)) * h * h;
+            if (gyz > lim2) continue;
             int mx = 0;
-            for (int dx = 1; dx <= m; ++dx)
-                if (gyz + gap(dx) * gap(dx) <= lim * lim) mx = dx;
+            for (int dx = 1; dx <= mxmax; ++dx)
+                if (gyz + gap(dx) * gap(dx) * hx * hx <= lim2) mx = dx;
             Row r;
-            r.dy = (int8_t)dy;
-            r.dz = (int8_t)dz;
-            r.x0 = (int8_t)((dz == 0 && dy == 0) ? 0 : -mx);
-            r.x1 = (int8_t)mx;
+            r.dy = (int16_t)dy;
+            r.dz = (int16_t)dz;
+            r.x0 = (int16_t)((dz == 0 && dy == 0) ? 0 : -mx);
+            r.x1 = (int16_t)mx;
             rows.push_back(r);
         }
     return rows;
 }
 
 // Full stencil (both half spaces) as x-rows, for apex-centred kernels.
-std::vector<Row> full_stencil(double h, double rmax, int m)
+std::vector<Row> full_stencil(double h, double hx, double rmax, int m)
 {
-    std::vector<Row> half = half_stencil(h, rmax, m);
+    std::vector<Row> half = half_stencil(h, hx, rmax, m);
     std::vector<Row> rows;
     for (const Row& r : half) {
         if (r.dy == 0 && r.dz == 0) {
             Row c = r;
-            c.x0 = (int8_t)(-r.x1);
+            c.x0 = (int16_t)(-r.x1);
             rows.push_back(c);
         } else {
             rows.push_back(r);
             Row n = r;
-            n.dy = (int8_t)(-r.dy);
-            n.dz = (int8_t)(-r.dz);
+            n.dy = (int16_t)(-r.dy);
+            n.dz = (int16_t)(-r.dz);
             rows.push_back(n);
         }
     }
@@ -330,6 +335,7 @@ struct gs_ctx {
     GridDesc g{};
     bool grid_valid = false;
     double grid_h = 0.0;
+    int grid_sx = 1;
     DevBuf<int32_t> cell_of, cell_start, fill, chunk_nc, chunk_off, scan_tmp;
     DevBuf<double4> pw;
     DevBuf<float4> pl;
@@ -584,45 +590,48 @@ double round11(double h)
     return std::ldexp(std::ceil(std::ldexp(fr, 11)), ex - 11);
 }
 
-// Picks the cell edge: rmax / m, widened if the grid would exceed kMaxCells.
-// Picks the cell edge for a catalogue with bounding box bbox: rmax / m rounded up to 11 significant
-// bits (then k*h is exact in FP32 for |k| < 2^13), widened while the grid would exceed max_cells.
-double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells)
+// Picks the y/z cell edge for a catalogue with bounding box bbox: rmax / m rounded up to 11 significant
+// bits (then k*h is exact in FP32 for |k| < 2^13), widened while the grid, with x cells of edge h / sx,
+// would exceed max_cells.
+double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells, int sx)
 {
     int ex;
     double fr = std::frexp(rmax / m, &ex);
     double h = std::ldexp(std::ceil(std::ldexp(fr, 11)), ex - 11);
     for (;;) {
         int64_t nc = 1;
-        for (int k = 0; k < 3; ++k) nc *= (int64_t)std::floor((bbox[3 + k] - bbox[k]) / h) + 1;
+        for (int k = 0; k < 3; ++k) nc *= (int64_t)std::floor((bbox[3 + k] - bbox[k]) / (k == 0 ? h / sx : h)) + 1;
         if (nc <= max_cells) break;
         h = std::ldexp(std::ceil(std::ldexp(std::frexp(h * 1.25, &ex), 11)), ex - 11);
     }
     return h;
 }
 
-static double choose_h(const gs_ctx* c, double rmax, int m)
+static double choose_h(const gs_ctx* c, double rmax, int m, int sx)
 {
-    return choose_h_bbox(c->bbox, rmax, m, (int64_t)1 << 24);
+    return choose_h_bbox(c->bbox, rmax, m, (int64_t)1 << 24, sx);
 }
 
-// Grid of a catalogue (input-order xyz/w/buf in device memory) over its bounding box with cell edge
-// h, built into caller-provided device buffers; *nch receives the number of row chunks.
+// Grid of a catalogue (input-order xyz/w/buf in device memory) over its bounding box with cell edges
+// h (y, z) and h / sx (x), built into caller-provided device buffers; *nch receives the number of row
+// chunks.
 int grid_build_raw(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
-                   const double* bbox, double h, const GridBufs& b, int64_t max_cells, GridDesc* g_out,
+                   const double* bbox, double h, int sx, const GridBufs& b, int64_t max_cells, GridDesc* g_out,
                    int32_t* nch, hipStream_t st)
 {
     GridDesc g{};
     g.h = h;
+    g.hx = h / sx;
     int64_t ncell = 1;
     for (int k = 0; k < 3; ++k) {
         g.lo[k] = bbox[k];
-        g.dim[k] = (int32_t)std::floor((bbox[3 + k] - bbox[k]) / h) + 1;
+        g.dim[k] = (int32_t)std::floor((bbox[3 + k] - bbox[k]) / (k == 0 ? g.hx : h)) + 1;
         ncell *= g.dim[k];
     }
     if (ncell > max_cells) return fail(GS_ELIMIT, "grid too large (%lld cells)", (long long)ncell);
     g.ncell = (int32_t)ncell;
     g.h32 = (float)h;
+    g.hx32 = (float)g.hx;
     hipError_t e = launch_fill_u32(b.fill, ncell + 1, 0u, st);
     if (e == hipSuccess) e = launch_cell_count(xyz, n, g, b.cell_of, b.fill, st);
     if (e == hipSuccess) e = exclusive_scan(b.fill, b.cell_start, ncell + 1, b.scan_tmp, st);
@@ -641,12 +650,12 @@ int grid_build_raw(const double* xyz, const double* w, const int32_t* buf, int64
     return GS_OK;
 }
 
-int build_grid(gs_ctx* c, double h)
+int build_grid(gs_ctx* c, double h, int sx)
 {
-    if (c->grid_valid && c->grid_h == h) return GS_OK;
+    if (c->grid_valid && c->grid_h == h && c->grid_sx == sx) return GS_OK;
     const int64_t n = c->n_data + c->n_rand;
     int64_t ncell = 1;
-    for (int k = 0; k < 3; ++k) ncell *= (int64_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / h) + 1;
+    for (int k = 0; k < 3; ++k) ncell *= (int64_t)std::floor((c->bbox[3 + k] - c->bbox[k]) / (k == 0 ? h / sx : h)) + 1;
     if (ncell > (1 << 26)) return fail(GS_ELIMIT, "grid too large (%lld cells)", (long long)ncell);
     int rc;
     if ((rc = c->cell_of.reserve(n)) || (rc = c->cell_start.reserve(ncell + 1)) || (rc = c->fill.reserve(ncell + 1)) ||
@@ -659,10 +668,11 @@ int build_grid(gs_ctx* c, double h)
     GridDesc g;
     int32_t nch = 0;
     if ((rc = grid_build_raw(c->xyz.ptr, c->has_w ? c->w.ptr : nullptr, c->has_buf ? c->buf.ptr : nullptr, n,
-                             c->n_data, c->bbox, h, b, (int64_t)1 << 26, &g, &nch, c->stream)))
+                             c->n_data, c->bbox, h, sx, b, (int64_t)1 << 26, &g, &nch, c->stream)))
         return rc;
     c->g = g;
     c->grid_h = h;
+    c->grid_sx = sx;
     c->n_chunks = nch;
     c->grid_valid = true;
     return GS_OK;
@@ -681,6 +691,19 @@ int pair_cell_div(int64_t n, const double* bbox, double rmax)
         if (dens * h * h * h >= 32.0) best = m;
     }
     return best;
+}
+
+// x cells per y/z cell of the pair kernels' grid: finer x cells cut each stencil row to the x-range that
+// can reach rmax from the work item's apexes, at the cost of one more entry per cell in the cell index.
+int pair_cell_sx()
+{
+    const char* ev = getenv("GS_CELL_SX");      // tuning override (unset or empty: automatic)
+    if (ev && *ev) {
+        int v = atoi(ev), sx = 1;
+        while (sx < 64 && sx * 2 <= v) sx *= 2;
+        return sx;
+    }
+    return 8;
 }
 
 int default_cell_div(const gs_ctx* c, double rmax, int mode)
@@ -792,13 +815,14 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
     const bool rsd = nm >= 2, fp64 = p->precision == GS_FP64, weighted = c->has_w;
     const int nbt = 3 * nb * nm;
     const int m = p->cell_div > 0 ? p->cell_div : default_cell_div(c, p->rmax, 2);
-    const double h = choose_h(c, p->rmax, m);
+    const int sx = pair_cell_sx();
+    const double h = choose_h(c, p->rmax, m, sx);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = half_stencil(h, p->rmax, mst);
+    std::vector<Row> rows = half_stencil(h, h / sx, p->rmax, mst);
     if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
 
     (void)hipEventRecord(c->ev[0], c->stream);
-    if ((rc = build_grid(c, h))) return rc;
+    if ((rc = build_grid(c, h, sx))) return rc;
     (void)hipEventRecord(c->ev[1], c->stream);
 
     double xmax = 0;
@@ -945,11 +969,11 @@ CtxView view(gs_ctx* c)
     v.buf = c->has_buf ? c->buf.ptr : nullptr;
     return v;
 }
-int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out)
+int ctx_build_grid(gs_ctx* c, double rmax, int m, int sx, double* h_out)
 {
-    double h = choose_h(c, rmax, m);
+    double h = choose_h(c, rmax, m, sx);
     *h_out = h;
-    return build_grid(c, h);
+    return build_grid(c, h, sx);
 }
 double ctx_xmax(gs_ctx* c)
 {

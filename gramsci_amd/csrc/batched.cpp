@@ -56,14 +56,16 @@ static bool same_params(const gs_params& a, const gs_params& b)
            a.log_bins == b.log_bins && a.precision == b.precision;
 }
 
-// Builds (or finds) the captured pipeline for parameters p at m cells per
-// rmax, ending in the pair kernel (kind 2) or the triplet kernel (kind 3).
-static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Plan** out)
+// Builds (or finds) the captured pipeline for parameters p with launch parameters tu (tu.m cells per
+// rmax), ending in the pair kernel (kind 2) or the triplet kernel (kind 3).
+static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind, GsBatch::Plan** out)
 {
     CtxView v = view(c);
     GsBatch& B = *v.batch;
+    const int m = tu.m;
+    const int sx_want = kind == 2 ? (tu.sx > 0 ? tu.sx : pair_cell_sx()) : 1;
     for (auto& pl : B.plans)
-        if (pl.m == m && pl.kind == kind && same_params(pl.p, p)) {
+        if (pl.tune == tu && pl.sx == sx_want && pl.kind == kind && same_params(pl.p, p)) {
             *out = &pl;
             return GS_OK;
         }
@@ -73,8 +75,10 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     const bool rsd = nm >= 2, fp64 = p.precision == GS_FP64, weighted = B.has_w;
     const int nbt = 3 * nb * nm;
 
-    // cell edge and per-region grids
+    // cell edges (x cut finer for the pair kernel) and per-region grids
     const double h = round11(p.rmax / m);
+    const int sx = sx_want;
+    const double hx = h / sx;
     std::vector<RegionDesc> reg(nreg);
     std::vector<int32_t> row_base(nreg + 1, 0);
     int64_t cells = 0, rows_tot = 0;
@@ -84,7 +88,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         int64_t nc = 1;
         for (int k = 0; k < 3; ++k) {
             d.lo[k] = B.bbox[6 * r + k];
-            d.dim[k] = (int32_t)std::floor((B.bbox[6 * r + 3 + k] - B.bbox[6 * r + k]) / h) + 1;
+            d.dim[k] = (int32_t)std::floor((B.bbox[6 * r + 3 + k] - B.bbox[6 * r + k]) / (k == 0 ? hx : h)) + 1;
             nc *= d.dim[k];
         }
         d.cell_base = (int32_t)cells;
@@ -101,7 +105,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     }
     row_base[nreg] = (int32_t)rows_tot;
     const int mst = (int)std::ceil(p.rmax / h - 1e-12);
-    std::vector<Row> rows = kind == 3 ? full_stencil(h, p.rmax, mst) : half_stencil(h, p.rmax, mst);
+    std::vector<Row> rows = kind == 3 ? full_stencil(h, hx, p.rmax, mst) : half_stencil(h, hx, p.rmax, mst);
     if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
     BinDesc bins;
     int rc;
@@ -186,6 +190,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     a.out_stats = B.out.ptr + (size_t)nreg * 3 * nbt;
     a.g.h = h;
     a.g.h32 = (float)h;
+    a.g.hx = hx;
+    a.g.hx32 = (float)hx;
     a.g.m = mst;
     a.nrows = (int)rows.size();
     a.reps = 1;
@@ -214,6 +220,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         ta.out_stats = B.out.ptr + (size_t)nreg * 3 * nh;
         ta.g.h = h;
         ta.g.h32 = (float)h;
+        ta.g.hx = hx;
+        ta.g.hx32 = (float)hx;
         ta.g.m = mst;
         ta.nrows = (int)rows.size();
         ta.n_items = (int)n;
@@ -242,12 +250,13 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
             for (int k = 0; k < 3; ++k) vol *= std::max(B.bbox[6 * r + 3 + k] - B.bbox[6 * r + k], p.rmax);
             kmean = std::max(kmean, (double)(B.off[r + 1] - B.off[r]) * shell / vol);
         }
-        ta1.wg = triplet_wg(kmean);
+        ta1.wg = tu.wg > 0 ? tu.wg : triplet_wg(kmean);
         const char* evl = getenv("GS_T3_KMAX_LDS");     // test override: a shorter second-pass list
         if (evl && *evl) ta.kmax = ta1.kmax = ta2.kmax = std::max(64, std::min(ta.kmax, atoi(evl)));
         int kmax1 = std::min(ta.kmax, std::max(128, (int)std::ceil(2.0 * kmean / 64.0) * 64));
         const char* ev = getenv("GS_T3_KMAX");      // tuning override (unset or empty: automatic)
-        if (ev && *ev) kmax1 = std::max(64, std::min(ta.kmax, atoi(ev)));
+        if (tu.kmax1 > 0) kmax1 = std::max(64, std::min(ta.kmax, tu.kmax1));
+        else if (ev && *ev) kmax1 = std::max(64, std::min(ta.kmax, atoi(ev)));
         if (kmax1 < ta.kmax) {
             ta1.kmax = kmax1;
             ta1.ovf_n = B.ovf.ptr;
@@ -274,12 +283,12 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     if (e == hipSuccess && kind == 3) e = launch_fill_u32(B.ovf.ptr + n + 1, 2, 0, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.out.ptr, (int64_t)nout * 2, 0, v.stream);
     if (e == hipSuccess)
-        e = launch_cell_count_b(B.xyz.ptr, n, B.d_off.ptr, nreg, d_reg, h, B.cell_of.ptr, B.count.ptr, v.stream);
+        e = launch_cell_count_b(B.xyz.ptr, n, B.d_off.ptr, nreg, d_reg, h, hx, B.cell_of.ptr, B.count.ptr, v.stream);
     if (e == hipSuccess) e = exclusive_scan(B.count.ptr, B.cell_start.ptr, cells + 1, B.scan_tmp.ptr, v.stream);
     if (e == hipSuccess) e = launch_fill_u32(B.count.ptr, cells + 1, 0, v.stream);
     if (e == hipSuccess)
         e = launch_scatter_b(B.xyz.ptr, B.has_w ? B.w.ptr : nullptr, B.has_buf ? B.buf.ptr : nullptr, n, B.d_off.ptr,
-                             nreg, d_reg, h, B.cell_of.ptr, B.cell_start.ptr,
+                             nreg, d_reg, h, hx, B.cell_of.ptr, B.cell_start.ptr,
                              B.count.ptr, B.pw.ptr, B.pl.ptr, B.flags.ptr, B.orig.ptr, v.stream);
     if (kind == 2) {
         if (e == hipSuccess)
@@ -301,7 +310,9 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
         return hip_fail(e, "batched graph capture");
     }
     P.p = p;
+    P.tune = tu;
     P.m = m;
+    P.sx = sx;
     P.graph = graph;
     P.exec = exec;
     P.nbt = nbt;
@@ -319,6 +330,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, int m, int kind, GsBatch::Pla
     P.nh = nh;
     if (kind == 3) {
         P.kmax = ta.kmax;
+        P.wg1 = ta1.wg;
+        P.kmax1 = ta1.kmax;
         P.t3 = ta;              // tables, outputs and regions of the device-memory pass
         P.t3_list = B.ovf.ptr + n + 3;
         P.t3_cnt = B.ovf.ptr + n + 1;
@@ -440,6 +453,104 @@ extern "C" int gs_set_catalogue_batched(gs_ctx* c, const double* xyz, const doub
     return GS_OK;
 }
 
+// Times a plan on this batch: one replay to warm up, one timed.
+static int time_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind, float* ms, GsBatch::Plan** out)
+{
+    GsBatch::Plan* P;
+    int rc;
+    if ((rc = get_plan(c, p, tu, kind, &P))) return rc;
+    if ((rc = run_plan(c, *P, nullptr)) || (rc = run_plan(c, *P, ms))) return rc;
+    *out = P;
+    return GS_OK;
+}
+
+// Launch parameters for parameter set p on this batch: the cached winner, or else the autotuner's.  The
+// tuner times the captured pipeline by coordinate descent: cells per rmax (pair 2..6, triplet 1..3),
+// then for the pair kernel the x cells per y/z cell (1, 2, 4, 8, 16), for the triplet kernel the
+// threads per work-group (64, 128, 256) and then the first-pass list length (half and twice the
+// automatic one).  Plans of the losing candidates are released; the winner's is kept.
+static int batch_tune(gs_ctx* c, const gs_params& p, int kind, BatchTune* out)
+{
+    GsBatch& B = *view(c).batch;
+    const std::string key = plan_key(p, kind);
+    for (auto& kv : B.tuned)
+        if (kv.first == key) { *out = kv.second; return GS_OK; }
+    BatchTune best{};
+    float tbest = 1e30f;
+    GsBatch::Plan* pbest = nullptr;
+    const char* ev_log = getenv("GS_TUNE_LOG");      // diagnostics: each candidate's time on stderr
+    const bool log = ev_log && *ev_log && *ev_log != '0';
+    auto trial = [&](const BatchTune& t) -> int {
+        for (auto& pl : B.plans)        // already timed
+            if (pl.kind == kind && same_params(pl.p, p) && pl.tune == t) return GS_OK;
+        float ms = 0;
+        GsBatch::Plan* P = nullptr;
+        int rc = time_plan(c, p, t, kind, &ms, &P);
+        if (rc) return rc;
+        if (log)
+            fprintf(stderr, "[gs tune] %dPCF m %d sx %d wg %d kmax1 %d: %.3f ms\n", kind, P->m, P->sx, P->wg1,
+                    P->kmax1, ms);
+        if (ms < tbest) { tbest = ms; best = t; }
+        return GS_OK;
+    };
+    auto best_plan = [&]() -> GsBatch::Plan* {
+        for (auto& pl : B.plans)
+            if (pl.kind == kind && same_params(pl.p, p) && pl.tune == best) return &pl;
+        return nullptr;
+    };
+    int rc;
+    for (int m = kind == 2 ? 2 : 1; m <= (kind == 2 ? 6 : 3); ++m) {
+        BatchTune t{};
+        t.m = m;
+        if ((rc = trial(t))) return rc;
+    }
+    if (kind == 2) {
+        const BatchTune b0 = best;
+        for (int sx : {1, 2, 4, 8, 16}) {
+            if (sx == pair_cell_sx()) continue;     // the automatic choice, timed above
+            BatchTune t = b0;
+            t.sx = sx;
+            if ((rc = trial(t))) return rc;
+        }
+    } else {
+        BatchTune b0 = best;
+        pbest = best_plan();
+        const int wg_auto = pbest ? pbest->wg1 : kWG;
+        for (int wg : {64, 128, 256}) {
+            if (wg == wg_auto) continue;
+            BatchTune t = b0;
+            t.wg = wg;
+            if ((rc = trial(t))) return rc;
+        }
+        b0 = best;
+        pbest = best_plan();
+        if (pbest) {
+            const int k1 = pbest->kmax1, kx = pbest->kmax;
+            for (int cand : {k1 / 2 / 64 * 64, std::min(kx, k1 * 2)}) {
+                if (cand < 64 || cand == k1) continue;
+                BatchTune t = b0;
+                t.kmax1 = cand;
+                if ((rc = trial(t))) return rc;
+            }
+        }
+    }
+    // keep only the winner's plan among this parameter set's
+    for (size_t k = 0; k < B.plans.size();) {
+        GsBatch::Plan& pl = B.plans[k];
+        if (pl.kind == kind && same_params(pl.p, p) && !(pl.tune == best)) {
+            if (pl.exec) (void)hipGraphExecDestroy(pl.exec);
+            if (pl.graph) (void)hipGraphDestroy(pl.graph);
+            pl.blob.release();
+            B.plans.erase(B.plans.begin() + k);
+        } else {
+            ++k;
+        }
+    }
+    B.tuned.emplace_back(key, best);
+    *out = best;
+    return GS_OK;
+}
+
 extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cnt, double* wsum, double* n2,
                                      double* n3, gs_stats* st)
 {
@@ -453,26 +564,11 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
     const int nreg = B.nreg, nb = p->nbins, nm = p->nmu, nbt = 3 * nb * nm;
     const bool weighted = B.has_w;
 
-    int m = p->cell_div;
-    if (m == 0) {
-        const std::string key = plan_key(*p, 2);
-        for (auto& kv : B.tuned)
-            if (kv.first == key) m = kv.second;
-        if (m == 0) {
-            // autotune: time the captured pipeline for each candidate on this batch
-            float best = 1e30f;
-            for (int cand = 2; cand <= 6; ++cand) {
-                GsBatch::Plan* P;
-                if ((rc = get_plan(c, *p, cand, 2, &P))) return rc;
-                float t = 0;
-                if ((rc = run_plan(c, *P, nullptr)) || (rc = run_plan(c, *P, &t))) return rc;
-                if (t < best) { best = t; m = cand; }
-            }
-            B.tuned.emplace_back(key, m);
-        }
-    }
+    BatchTune tu{};
+    tu.m = p->cell_div;
+    if (tu.m == 0 && (rc = batch_tune(c, *p, 2, &tu))) return rc;
     GsBatch::Plan* P;
-    if ((rc = get_plan(c, *p, m, 2, &P))) return rc;
+    if ((rc = get_plan(c, *p, tu, 2, &P))) return rc;
     float ms = 0;
     if ((rc = run_plan(c, *P, &ms))) return rc;
     const size_t nout = (size_t)nreg * 3 * nbt + 4;
@@ -533,25 +629,11 @@ extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
     const int ncfg = nb * (nb + 1) * (nb + 2) / 6;
     const bool weighted = B.has_w;
 
-    int m = p->cell_div;
-    if (m == 0) {
-        const std::string key = plan_key(*p, 3);
-        for (auto& kv : B.tuned)
-            if (kv.first == key) m = kv.second;
-        if (m == 0) {
-            float best = 1e30f;
-            for (int cand = 1; cand <= 3; ++cand) {
-                GsBatch::Plan* P;
-                if ((rc = get_plan(c, *p, cand, 3, &P))) return rc;
-                float t = 0;
-                if ((rc = run_plan(c, *P, nullptr)) || (rc = run_plan(c, *P, &t))) return rc;
-                if (t < best) { best = t; m = cand; }
-            }
-            B.tuned.emplace_back(key, m);
-        }
-    }
+    BatchTune tu{};
+    tu.m = p->cell_div;
+    if (tu.m == 0 && (rc = batch_tune(c, *p, 3, &tu))) return rc;
     GsBatch::Plan* P;
-    if ((rc = get_plan(c, *p, m, 3, &P))) return rc;
+    if ((rc = get_plan(c, *p, tu, 3, &P))) return rc;
     float ms = 0;
     if ((rc = run_plan(c, *P, &ms))) return rc;
     const int nh = P->nh;
@@ -583,7 +665,8 @@ extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         st->n_recheck = s[1];
         st->n_overflow = s[2];
         st->n_units = s[3];
-        st->n_binned = s[4];
+        st->n_binned = 0;           // the binned triplets are the counts' total
+        for (size_t k = 0; k < (size_t)nreg * nh; ++k) st->n_binned += h[k];
         st->cell_div = P->mst;
     }
     return GS_OK;

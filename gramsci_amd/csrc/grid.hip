@@ -101,9 +101,9 @@ __device__ __forceinline__ int cell_coord(double v, double lo, double inv_h, int
 __global__ __launch_bounds__(256) void k_cell_count(const double* __restrict__ xyz, int64_t n, GridDesc g,
                                                     int32_t* __restrict__ cell_of, int32_t* __restrict__ count)
 {
-    const double inv_h = 1.0 / g.h;
+    const double inv_h = 1.0 / g.h, inv_hx = 1.0 / g.hx;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int cx = cell_coord(xyz[3 * i + 0], g.lo[0], inv_h, g.dim[0]);
+        int cx = cell_coord(xyz[3 * i + 0], g.lo[0], inv_hx, g.dim[0]);
         int cy = cell_coord(xyz[3 * i + 1], g.lo[1], inv_h, g.dim[1]);
         int cz = cell_coord(xyz[3 * i + 2], g.lo[2], inv_h, g.dim[2]);
         int c = (cz * g.dim[1] + cy) * g.dim[0] + cx;
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz,
         pw[pos] = make_double4(x, y, z, w ? w[i] : 1.0);
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
         // cell-local FP32 coordinates; .w carries the cell's x index (exact in FP32)
-        pl[pos] = make_float4((float)(x - (g.lo[0] + cx * g.h)), (float)(y - (g.lo[1] + cy * g.h)),
+        pl[pos] = make_float4((float)(x - (g.lo[0] + cx * g.hx)), (float)(y - (g.lo[1] + cy * g.h)),
                               (float)(z - (g.lo[2] + cz * g.h)), (float)cx);
         const uint32_t f = (i >= n_data ? 1u : 0u) | ((buf && buf[i] == 1) ? 2u : 0u);
         flags[pos] = f;
@@ -298,13 +298,13 @@ __device__ __forceinline__ int region_of(const int32_t* off, int nreg, int64_t i
 
 __global__ __launch_bounds__(256) void k_cell_count_b(const double* __restrict__ xyz, int64_t n,
                                                       const int32_t* __restrict__ off, int nreg,
-                                                      const RegionDesc* __restrict__ reg, double h,
+                                                      const RegionDesc* __restrict__ reg, double h, double hx,
                                                       int32_t* __restrict__ cell_of, int32_t* __restrict__ count)
 {
-    const double inv_h = 1.0 / h;
+    const double inv_h = 1.0 / h, inv_hx = 1.0 / hx;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const RegionDesc R = reg[region_of(off, nreg, i)];
-        int cx = cell_coord(xyz[3 * i + 0], R.lo[0], inv_h, R.dim[0]);
+        int cx = cell_coord(xyz[3 * i + 0], R.lo[0], inv_hx, R.dim[0]);
         int cy = cell_coord(xyz[3 * i + 1], R.lo[1], inv_h, R.dim[1]);
         int cz = cell_coord(xyz[3 * i + 2], R.lo[2], inv_h, R.dim[2]);
         int c = R.cell_base + (cz * R.dim[1] + cy) * R.dim[0] + cx;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void k_cell_count_b(const double* __restrict__
 __global__ __launch_bounds__(256) void k_scatter_b(const double* __restrict__ xyz, const double* __restrict__ w,
                                                    const int32_t* __restrict__ buf, int64_t n,
                                                    const int32_t* __restrict__ off, int nreg,
-                                                   const RegionDesc* __restrict__ reg, double h,
+                                                   const RegionDesc* __restrict__ reg, double h, double hx,
                                                    const int32_t* __restrict__ cell_of,
                                                    const int32_t* __restrict__ start, int32_t* __restrict__ fill,
                                                    double4* __restrict__ pw, float4* __restrict__ pl,
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void k_scatter_b(const double* __restrict__ xy
         pw[pos] = make_double4(x, y, z, w ? w[i] : 1.0);
         const int lc = c - R.cell_base;
         const int cx = lc % R.dim[0], cy = (lc / R.dim[0]) % R.dim[1], cz = lc / (R.dim[0] * R.dim[1]);
-        pl[pos] = make_float4((float)(x - (R.lo[0] + cx * h)), (float)(y - (R.lo[1] + cy * h)),
+        pl[pos] = make_float4((float)(x - (R.lo[0] + cx * hx)), (float)(y - (R.lo[1] + cy * h)),
                               (float)(z - (R.lo[2] + cz * h)), (float)cx);
         const uint32_t f = ((i - R.pt_base) >= R.n_data ? 1u : 0u) | ((buf && buf[i] == 1) ? 2u : 0u);
         flags[pos] = f;
@@ -379,19 +379,20 @@ hipError_t launch_fill_u32(void* p, int64_t n, uint32_t v, hipStream_t st)
 }
 
 hipError_t launch_cell_count_b(const double* xyz, int64_t n, const int32_t* off, int nreg, const RegionDesc* reg,
-                               double h, int32_t* cell_of, int32_t* count, hipStream_t st)
+                               double h, double hx, int32_t* cell_of, int32_t* count, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cell_count_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, n, off, nreg, reg, h, cell_of,
+    hipLaunchKernelGGL(k_cell_count_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, n, off, nreg, reg, h, hx, cell_of,
                        count);
     return hipGetLastError();
 }
 
 hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* buf, int64_t n, const int32_t* off,
-                            int nreg, const RegionDesc* reg, double h, const int32_t* cell_of, const int32_t* start,
+                            int nreg, const RegionDesc* reg, double h, double hx, const int32_t* cell_of,
+                            const int32_t* start,
                             int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
 {
     for (int pass = 0; pass < 2; ++pass)
-        hipLaunchKernelGGL(k_scatter_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, w, buf, n, off, nreg, reg, h,
+        hipLaunchKernelGGL(k_scatter_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, w, buf, n, off, nreg, reg, h, hx,
                            cell_of, start, fill, pw, pl, flags, orig, pass);
     return hipGetLastError();
 }

@@ -37,6 +37,14 @@ void extra_release(GsExtra& x);
 
 // Batched independent regions (batched.cpp): host description of the regions
 // and the device buffers / captured hipGraphs of the batched pipeline.
+// Launch parameters of a batched plan that the autotuner times (0: the automatic choice): cells per
+// rmax in y/z (m, required), x cells per y/z cell (pair kernel), threads per work-group and first-pass
+// neighbour-list length (triplet kernel).
+struct BatchTune {
+    int m = 0, sx = 0, wg = 0, kmax1 = 0;
+    bool operator==(const BatchTune& o) const { return m == o.m && sx == o.sx && wg == o.wg && kmax1 == o.kmax1; }
+};
+
 struct GsBatch {
     int nreg = 0;
     bool valid = false;
@@ -59,7 +67,10 @@ struct GsBatch {
     bool has_w = false, has_buf = false;
     struct Plan {
         gs_params p{};
+        BatchTune tune{};                // as requested (0: automatic)
         int m = 0;
+        int sx = 1;                      // x cells per y/z cell
+        int wg1 = 0, kmax1 = 0;          // triplet first pass: threads per work-group, list length
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int nbt = 0;
@@ -79,7 +90,7 @@ struct GsBatch {
         bool fp64 = false, weighted = false;
     };
     std::vector<Plan> plans;
-    std::vector<std::pair<std::string, int>> tuned;   // parameter key -> tuned cells per rmax
+    std::vector<std::pair<std::string, BatchTune>> tuned;   // parameter key -> tuned launch parameters
 };
 void batch_release(GsBatch& b);
 
@@ -111,7 +122,7 @@ CtxView view(gs_ctx* c);
 int ctx_set_device(gs_ctx* c);
 int ctx_upload(gs_ctx* c, const double* xyz, const double* w, const int32_t* buf, int64_t n);
 double round11(double h);
-int ctx_build_grid(gs_ctx* c, double rmax, int m, double* h_out);
+int ctx_build_grid(gs_ctx* c, double rmax, int m, int sx, double* h_out);
 // device buffers of a grid build (sizes: cell_of, pw, pl, flags, orig n; cell_start, fill, chunk_nc,
 // chunk_off ncell + 1; scan_tmp scan_scratch_size(ncell + 1))
 struct GridBufs {
@@ -122,11 +133,12 @@ struct GridBufs {
     int32_t* orig;
 };
 int grid_build_raw(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
-                   const double* bbox, double h, const GridBufs& b, int64_t max_cells, GridDesc* g_out,
+                   const double* bbox, double h, int sx, const GridBufs& b, int64_t max_cells, GridDesc* g_out,
                    int32_t* nch, hipStream_t st);
 int pair2_shape(int nb, int nm, bool fp64, bool weighted, bool logb, int* reps, int* wg, int* waves);
-double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells);
+double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells, int sx);
 int pair_cell_div(int64_t n, const double* bbox, double rmax);
+int pair_cell_sx();
 void combine2(const unsigned long long* h_out, int nb, int nm, const BinDesc& B, bool weighted, long double Wd,
               long double Wr, uint64_t* cnt, double* wsum, double* n2, double* n3, gs_stats* st);
 int check_params(const gs_params* p, int mode);
@@ -139,8 +151,8 @@ int fx_shift_triples(double wmax);
 inline int fx_pair_class_shift(const BinDesc& B, int cls) { return cls == 0 ? 2 * B.fx_sd : cls == 1 ? B.fx_sd + B.fx_sr : 2 * B.fx_sr; }
 // sum of w[0..n) as the exact sum of the fixed-point integers wsum_fixed(w[i], k), as a long double
 long double fixed_sum(const double* w, int64_t n, int k);
-std::vector<Row> half_stencil(double h, double rmax, int m);
-std::vector<Row> full_stencil(double h, double rmax, int m);
+std::vector<Row> half_stencil(double h, double hx, double rmax, int m);
+std::vector<Row> full_stencil(double h, double hx, double rmax, int m);
 double ctx_xmax(gs_ctx* c);
 long double fx_value(unsigned long long lo, unsigned long long hi, int shift);
 int default_cell_div(const gs_ctx* c, double rmax, int mode);

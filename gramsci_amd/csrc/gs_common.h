@@ -12,16 +12,20 @@ constexpr int kWG = 256;        // work-group size of the counting kernels
 constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kChunk = 64;      // points per work item of the pair kernel (one per lane)
 
-// Uniform cell grid over the catalogue's bounding box.  Points are stored
-// sorted by cell id = (cz*ny + cy)*nx + cx (x fastest), so one x-row of cells
-// is one contiguous range of points.
+// Cell grid over the catalogue's bounding box.  Points are stored sorted by
+// cell id = (cz*ny + cy)*nx + cx (x fastest), so one x-row of cells is one
+// contiguous range of points.  Cells have edge h in y and z and hx = h / sx in
+// x (sx a power of two): a row cut finely in x lets the pair kernel take, per
+// stencil row, just the x-range that can reach rmax from its chunk of apexes.
 struct GridDesc {
     double lo[3];
-    double h;                // cell edge (a float with <= 11 significant bits, so k*h is exact in FP32)
-    int32_t dim[3];
+    double h;                // y/z cell edge (a float with <= 11 significant bits, so k*h is exact in FP32)
+    double hx;               // x cell edge, h / sx (exact)
+    int32_t dim[3];          // dim[0] counts cells of edge hx
     int32_t ncell;
-    int32_t m;               // cells per rmax
+    int32_t m;               // y/z cells per rmax
     float h32;               // h as float (exact)
+    float hx32;              // hx as float (exact)
 };
 
 // Binning constants.  t64 are the exact FP64 thresholds on s = |xj - xi|^2
@@ -84,9 +88,9 @@ __host__ __device__ inline void wsum_fixed(double w, int k, uint64_t& lo, uint64
     hi = (uint64_t)h;
 }
 
-// Half-stencil rows of the pair kernel: cells (ax+dx, ay+dy, az+dz) with dx in [x0, x1].
+// Stencil rows: cells (ax+dx, ay+dy, az+dz) with dx in [x0, x1] (x in cells of edge hx).
 struct Row {
-    int8_t dy, dz, x0, x1;
+    int16_t dy, dz, x0, x1;
 };
 constexpr int kMaxRows = 2048;
 

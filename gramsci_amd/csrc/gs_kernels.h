@@ -26,9 +26,10 @@ hipError_t launch_row_chunks(const int32_t* start, int nx, int nrow, int32_t* nc
 // batched regions (grid.hip)
 hipError_t launch_fill_u32(void* p, int64_t n, uint32_t v, hipStream_t st);
 hipError_t launch_cell_count_b(const double* xyz, int64_t n, const int32_t* off, int nreg, const RegionDesc* reg,
-                               double h, int32_t* cell_of, int32_t* count, hipStream_t st);
+                               double h, double hx, int32_t* cell_of, int32_t* count, hipStream_t st);
 hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* buf, int64_t n, const int32_t* off,
-                            int nreg, const RegionDesc* reg, double h, const int32_t* cell_of, const int32_t* start,
+                            int nreg, const RegionDesc* reg, double h, double hx, const int32_t* cell_of,
+                            const int32_t* start,
                             int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
 hipError_t launch_row_chunks_b(const int32_t* start, const RegionDesc* reg, const int32_t* row_base, int nreg,
                                int nrow_tot, int32_t* nch, hipStream_t st);

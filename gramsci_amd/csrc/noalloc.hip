@@ -155,9 +155,10 @@ extern "C" int gs_count_2pcf_noalloc(gs_ctx* c, const gs_params* p, const double
     const bool rsd = nm >= 2, fp64 = p->precision == GS_FP64, weighted = d_w != nullptr;
     const int nbt = 3 * nb * nm;
     const int m = p->cell_div > 0 ? p->cell_div : pair_cell_div(n, bbox, p->rmax);
-    const double h = choose_h_bbox(bbox, p->rmax, m, kNoAllocMaxCells);
+    const int sx = pair_cell_sx();
+    const double h = choose_h_bbox(bbox, p->rmax, m, kNoAllocMaxCells, sx);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = half_stencil(h, p->rmax, mst);
+    std::vector<Row> rows = half_stencil(h, h / sx, p->rmax, mst);
     if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
     GridBufs gb{(int32_t*)(base + kOffCellOf), (int32_t*)(base + kOffCellStart), (int32_t*)(base + kOffFill),
                 (int32_t*)(base + kOffChunkNc), (int32_t*)(base + kOffChunkOff), (int32_t*)(base + kOffScan),
@@ -168,9 +169,9 @@ extern "C" int gs_count_2pcf_noalloc(gs_ctx* c, const gs_params* p, const double
     int32_t n_chunks = 0;
     if (n > 0) {
         int64_t ncell = 1;
-        for (int k = 0; k < 3; ++k) ncell *= (int64_t)std::floor((bbox[3 + k] - bbox[k]) / h) + 1;
+        for (int k = 0; k < 3; ++k) ncell *= (int64_t)std::floor((bbox[3 + k] - bbox[k]) / (k == 0 ? h / sx : h)) + 1;
         if (scan_scratch_size(ncell + 1) > kScanTmpInts) return fail(GS_ELIMIT, "scan scratch too small");
-        if ((rc = grid_build_raw(d_xyz, d_w, d_buf, n, n_data, bbox, h, gb, kNoAllocMaxCells, &g, &n_chunks, s)))
+        if ((rc = grid_build_raw(d_xyz, d_w, d_buf, n, n_data, bbox, h, sx, gb, kNoAllocMaxCells, &g, &n_chunks, s)))
             return rc;
     }
     (void)hipEventRecord(v.ev[1], s);

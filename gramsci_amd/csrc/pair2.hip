@@ -214,7 +214,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     int sd = B.fx_sd, sr = B.fx_sr;
     const int nrow_all = batched ? 0 : a.g.dim[1] * a.g.dim[2];
     const int n_chunks = batched ? *a.n_chunks_dev : a.n_chunks;
-    const float h32 = a.g.h32;
+    const float h32 = a.g.h32, hx32 = a.g.hx32;    // cell edges in y/z and in x
     int cur_r = -1;                 // region of this wave's private histogram (batched mode)
 
     Stats st{0, 0, 0};
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         const int jmin4 = jmin << 2;
                         for (int k = 0; k < (skip ? 0 : P); ++k) {
                             if (vi && ep >= jmin4) do {
-                                const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
+                                const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;
                                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                                 if (!(s > rej_lo && s < rej_hi)) break;
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         // product rides along).  eval32 bins the pair of this lane's point and tile
                         // entry eidx, with the guard band and the FP64 recheck.
                         auto eval32 = [&](const float4 e, const uint32_t ef, const int eidx) {
-                            const float dx = fmaf(e.w - Li.w, h32, e.x - Li.x);
+                            const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
                             const float dy = e.y - yi_r, dz = e.z - zi_r;
                             const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                             if (!(s > rej_lo && s < rej_hi)) return;

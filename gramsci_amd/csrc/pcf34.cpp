@@ -241,11 +241,11 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
 
     (void)hipEventRecord(v.ev[0], v.stream);
     double h;
-    if ((rc = ctx_build_grid(c, p->rmax, m, &h))) return rc;
+    if ((rc = ctx_build_grid(c, p->rmax, m, 1, &h))) return rc;
     (void)hipEventRecord(v.ev[1], v.stream);
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = full_stencil(h, p->rmax, mst);
+    std::vector<Row> rows = full_stencil(h, h, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
     std::vector<uint16_t> tab = equi ? std::vector<uint16_t>() : table3(nb);
@@ -407,6 +407,10 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     }
     combine3(S.data(), ncfg * nm, v.wsum_d, v.wsum_r, !(equi && nm > 1), n2, n3);
     fill_stats(st, v.ev, h_out.data() + 3 * nh, mst);
+    if (st) {
+        st->n_binned = 0;           // the binned triplets are the counts' total
+        for (int k = 0; k < nh; ++k) st->n_binned += h_out[k];
+    }
     return GS_OK;
 }
 
@@ -446,11 +450,11 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
 
     (void)hipEventRecord(v.ev[0], v.stream);
     double h;
-    if ((rc = ctx_build_grid(c, p->rmax, m, &h))) return rc;
+    if ((rc = ctx_build_grid(c, p->rmax, m, 1, &h))) return rc;
     (void)hipEventRecord(v.ev[1], v.stream);
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = full_stencil(h, p->rmax, mst);
+    std::vector<Row> rows = full_stencil(h, h, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
     std::vector<uint16_t> tab = tetra ? std::vector<uint16_t>() : table4(nb);

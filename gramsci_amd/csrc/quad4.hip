@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     __syncthreads();
 
     const int nxg = a.g.dim[0], nyg = a.g.dim[1], nzg = a.g.dim[2];
-    const double inv_h = 1.0 / a.g.h;
+    const double inv_h = 1.0 / a.g.h, inv_hx = 1.0 / a.g.hx;
     unsigned long long units = 0, binned = 0, over = 0, cand = 0;
     const int xcd = blockIdx.x & 7;
     int tried = 0;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             continue;
         }
         const double4 Pa = a.pw[apex];
-        const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_h), 0), nxg - 1);
+        const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_hx), 0), nxg - 1);
         const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), nyg - 1);
         const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nzg - 1);
 

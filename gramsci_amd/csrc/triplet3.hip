@@ -21,8 +21,9 @@
 //   1. gather: the work-group scans the apex's full stencil of cell rows and
 //      appends every neighbour with rmin < d < rmax to an LDS list, with its
 //      exact FP64 bin (side 1 / side 2 of the reference);
-//   2. pairs: the K(K-1)/2 neighbour pairs are spread evenly over the 256
-//      lanes (flat pair index, advanced by 256 per step); the third side is
+//   2. pairs: the K(K-1)/2 neighbour pairs, folded into rows of K, go to the
+//      waves row by row or, where rows would leave lanes idle, as one flat
+//      sequence interleaved over the work-group; the third side is
 //      evaluated directly from the neighbours' coordinates - in FP64, or in
 //      FP32 relative to the apex with the same guard band and FP64 recheck as
 //      the pair kernel - instead of a binary search in a neighbour list.
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
     int shift = B.fx_shift3;         // batched: the apex's region's scale
-    const double inv_h = 1.0 / a.g.h;
+    const double inv_h = 1.0 / a.g.h, inv_hx = 1.0 / a.g.hx;
     const bool batched = GEN && a.nreg > 0;
     int cur_r = batched ? -1 : 0;   // region whose histogram the LDS holds (work-group uniform)
 
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         }
     };
 
-    unsigned long long units = 0, binned = 0, rechk = 0, over = 0, cand = 0;
+    unsigned long long units = 0, rechk = 0, over = 0, cand = 0;   // binned triplets: the histogram's total
     unsigned long long bound = 0;   // u32 increments since the last drain (uniform)
     const int xcd = blockIdx.x & 7;
     int tried = 0;
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             }
         }
         const double4 Pa = a.pw[apex];
-        const int cx = min(max((int)floor((Pa.x - lx) * inv_h), 0), nx - 1);
+        const int cx = min(max((int)floor((Pa.x - lx) * inv_hx), 0), nx - 1);
         const int cy = min(max((int)floor((Pa.y - ly) * inv_h), 0), ny - 1);
         const int cz = min(max((int)floor((Pa.z - lz) * inv_h), 0), nz - 1);
 
@@ -284,30 +285,12 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         const double wa = Pa.w;
 
         // ---- 2. neighbour pairs
-        // The triangle u < v is folded into rows of K: pair index pi = r*K + c is
-        // (u, v) = (r, r+1+c) for c < K-1-r, else (K-2-r, c) (row K-2-r's pairs).
-        // A lane's index advances by the work-group size per step, i.e. c by that mod K with at most
-        // one carry into r: no loop, and neighbouring lanes read neighbouring entries.
-        const int Kc = max(K, 1);
-        const int dq = nt / Kc, dr = nt - dq * Kc;
-        int r = tid / Kc, c = tid - r * Kc;
-        // The pairs run in segments: one for an LDS list; for a device-memory list (whose apex
-        // may have so many neighbours that its pairs alone could overflow a u32 LDS count)
-        // segments of 2^30 pairs, with the LDS histogram drained after each.
-        constexpr pidx_t kSeg = GEN == 2 ? ((pidx_t)1 << 30) : (pidx_t)0;
-        pidx_t pi = tid;
-        for (pidx_t seg0 = 0; seg0 < P; seg0 += (GEN == 2 ? kSeg : P)) {
-        const pidx_t seg1 = GEN == 2 ? min(P, seg0 + kSeg) : P;
-        if (GEN == 2 && seg0 > 0) {
-            __syncthreads();
-            if (lh) drain(cur_r);
-            bound = 0;
-            __syncthreads();
-        }
-        for (; pi < seg1; pi += nt) {
-            const bool first = c < K - 1 - r;
-            const int u = first ? r : K - 2 - r;
-            const int v = first ? r + 1 + c : c;
+        // The triangle u < v is folded into rows: folded row r (0 <= r < K/2) holds the K pairs
+        // (r, r+1+c) for c < K-1-r and (K-2-r, c) for K-1-r <= c < K (row K-2-r's pairs); for
+        // even K the last folded row is row r alone (K/2 pairs).  A row is uniform across the
+        // lanes working on it, so each pair costs a compare and two selects for its (u, v), and
+        // neighbouring lanes read neighbouring entries.
+        auto visit = [&](const int u, const int v) {
             uint32_t pu, pv;
             int b3 = -1;
             if constexpr (FP64) {
@@ -363,7 +346,6 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 }
                 const int cls = ra + (int)((pu >> 14) & 1u) + (int)((pv >> 14) & 1u);
                 const int hk = (cfg * nm + mu) * 4 + cls;
-                ++binned;
                 if (lh) {
                     atomicAdd(&hc[(hk << rsh) + rep], 1u);
                     if constexpr (W) {
@@ -382,13 +364,75 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     }
                 }
             }
-            c += dr;
-            const bool carry = c >= Kc;
-            c -= carry ? Kc : 0;
-            r += dq + (carry ? 1 : 0);
+        };
+        const int R = K / 2;
+        const int wv = tid >> 6, ln = tid & 63, nw = nt >> 6;
+        if constexpr (GEN != 2) {
+            // Two schedules, chosen per apex (K is uniform in the work-group):
+            //  * rows, when they keep at least 7/8 of the lanes busy: wave w takes folded rows (groups of
+            //    G = 64 / K rows for K <= 64) w, w + nw, ...; its lanes take columns l, l + 64, ...
+            //    (lane l: row l / K, column l mod K for K <= 64).  Row interleaving balances the waves,
+            //    and the index arithmetic is one add and one compare per step;
+            //  * otherwise the flat schedule: the pair index pi = r K + c (0 <= pi < K(K-1)/2) starts at
+            //    the thread index and advances by the work-group size, i.e. c by that mod K and r by its
+            //    quotient, with at most one carry: no lane idles before the last step.
+            // In both, neighbouring lanes read neighbouring LDS entries.
+            const int G = (K > 0 && K <= 64) ? 64 / K : 1;
+            const int passes = K > 64 ? (K + 63) >> 6 : 1;
+            const bool by_rows = K > 64 ? K * 8 >= passes * 64 * 7 : G * K * 8 >= 64 * 7;
+            if (by_rows) {
+                const int rr = K <= 64 ? ln / K : 0;
+                const int c0 = K <= 64 ? ln - rr * K : ln;
+                const int grp = nw * G;
+                if (rr < G)
+                    for (int r = wv * G + rr; r < R; r += grp) {
+                        const int t = K - 1 - r;            // pairs of row r itself
+                        const int len = (2 * r + 2 == K) ? t : K;
+                        const int u2 = K - 2 - r;
+                        for (int c = c0; c < len; c += 64) {
+                            const bool first = c < t;
+                            visit(first ? r : u2, first ? r + 1 + c : c);
+                        }
+                    }
+            } else {
+                const int P = K * (K - 1) / 2;
+                const int dq = nt / K, dr = nt - dq * K;
+                int r = tid / K, c = tid - r * K;
+                for (int pi = tid; pi < P; pi += nt) {
+                    const bool first = c < K - 1 - r;
+                    visit(first ? r : K - 2 - r, first ? r + 1 + c : c);
+                    c += dr;
+                    const bool carry = c >= K;
+                    c -= carry ? K : 0;
+                    r += dq + (carry ? 1 : 0);
+                }
+            }
+        } else {
+            // Device-memory list (K beyond any LDS list): rows go to the waves in turn, wave w taking
+            // folded rows w, w + nw, ..., its lanes columns l, l + 64, ... of the row.  So many pairs
+            // alone could overflow a u32 LDS count: the rows run in blocks of fewer than 2^30 pairs,
+            // with the LDS histogram drained after each block (all waves), and after the apex.
+            const int blk = max(nw, (int)std::min<long long>((1ll << 30) / max(K, 1), (long long)R + nw) / nw * nw);
+            for (int b0 = 0; b0 < R; b0 += blk) {
+                if (b0 > 0) {
+                    __syncthreads();
+                    if (lh) drain(cur_r);
+                    bound = 0;
+                    __syncthreads();
+                }
+                const int b1 = min(R, b0 + blk);
+                for (int r = b0 + wv; r < b1; r += nw) {
+                    const int t = K - 1 - r;                // pairs of row r itself
+                    const int len = (2 * r + 2 == K) ? t : K;
+                    const int u2 = K - 2 - r;
+                    for (int c = ln; c < len; c += 64) {
+                        const bool first = c < t;
+                        visit(first ? r : u2, first ? r + 1 + c : c);
+                    }
+                }
+            }
         }
-        }
-        bound += (unsigned long long)(GEN == 2 ? (P - (P - 1) / kSeg * kSeg) : P);   // pairs since the last drain
+        bound += GEN == 2 ? (1ull << 31) + 1 : (unsigned long long)P;   // LDS increments since the last drain
         __syncthreads();
         if (lh && bound > (1ull << 31)) {
             drain(cur_r);
@@ -400,7 +444,6 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     if (lh && cur_r >= 0) drain(cur_r);
     for (int off = 32; off > 0; off >>= 1) {
         units += __shfl_down(units, off, 64);
-        binned += __shfl_down(binned, off, 64);
         rechk += __shfl_down(rechk, off, 64);
         over += __shfl_down(over, off, 64);
         cand += __shfl_down(cand, off, 64);
@@ -410,7 +453,6 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         if (rechk) atomicAdd(&a.out_stats[1], rechk);
         if (over) atomicAdd(&a.out_stats[2], over);
         if (units) atomicAdd(&a.out_stats[3], units);
-        if (binned) atomicAdd(&a.out_stats[4], binned);
     }
 }
 
@@ -469,9 +511,9 @@ __global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int n
     const BinDesc& B = *a.bins;
     const double t_lo = B.t64[0], t_hi = B.t64[B.nb + 1];
     const int nx = a.g.dim[0], ny = a.g.dim[1], nz = a.g.dim[2];
-    const double inv_h = 1.0 / a.g.h;
+    const double inv_h = 1.0 / a.g.h, inv_hx = 1.0 / a.g.hx;
     const double4 Pa = a.pw[apex];
-    const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_h), 0), nx - 1);
+    const int cx = min(max((int)floor((Pa.x - a.g.lo[0]) * inv_hx), 0), nx - 1);
     const int cy = min(max((int)floor((Pa.y - a.g.lo[1]) * inv_h), 0), ny - 1);
     const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nz - 1);
     int cnt = 0;

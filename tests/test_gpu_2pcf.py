@@ -143,6 +143,29 @@ def test_buffer_points(eng):
     _check(eng, xyz, None, 8000, p, buf=buf)
 
 
+@pytest.mark.parametrize("sx", [1, 2, 16, 64])
+def test_x_cell_subdivisions(eng, monkeypatch, sx):
+    # the grid's x cells are h / sx (GS_CELL_SX): counts, weighted sums and N2/N3 match the oracle for
+    # every subdivision (weighted, clustered, with buffer points, a lattice with exact edge distances),
+    # and finer x cells never examine more candidate pairs
+    monkeypatch.setenv("GS_CELL_SX", str(sx))
+    rng = np.random.default_rng(46)
+    d = C.clustered(6000, 120.0, 47)
+    r = C.uniform(12000, 120.0, 48)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(6000, 0.3, 49), np.ones(12000)])
+    buf = (rng.random(len(xyz)) < 0.2).astype(np.int32)
+    p = O.Params(1.0, 20.0, 8, 5, True)
+    print(f"sx={sx}", flush=True)
+    _, outs = _check(eng, xyz, w, 6000, p, buf=buf, cell_divs=(0, 3))
+    monkeypatch.setenv("GS_CELL_SX", "1")
+    eng.set_catalogue(xyz, w, 6000, buf)
+    base = eng.pcf2(p.rmin, p.rmax, p.nbins, p.nmu, p.log_bins, "fp32")
+    assert outs[2]["stats"]["n_candidates"] <= base["stats"]["n_candidates"]
+    monkeypatch.setenv("GS_CELL_SX", str(sx))
+    _check(eng, C.lattice(11, 1.0, -5.0), None, 11 ** 3, O.Params(1.0, 4.0, 3, 4, False))
+
+
 def test_lattice_edges_and_origin(eng):
     # exact-edge distances and mu values on a lattice that contains the origin
     # (mid = 0 pairs give theta = NaN in the reference)

@@ -216,6 +216,12 @@ def test_4pcf_config3_full_size_properties(eng, monkeypatch):
         assert np.array_equal(o["cnt"], a["cnt"])
         assert o["stats"]["n_units"] == a["stats"]["n_units"] and o["stats"]["n_binned"] == a["stats"]["n_binned"]
     assert a["stats"]["n_binned"] > 0
+    # and the oracle on the same 150k points (about 15 s of one CPU core): every quad count per
+    # (configuration, side bins, random pattern), the evaluated quads, N2 and N3
+    ref = O.pcf4(xyz, None, len(d), O.Params(10.0, 20.0, 3, 1, False))
+    assert np.array_equal(a["cnt"], ref["cnt"])
+    assert a["stats"]["n_units"] == ref["n_quads"]
+    assert _close(a["n2"], ref["n2"]) and _close(a["n3"], ref["n3"])
 
 
 @pytest.mark.parametrize("batch", ["7", "16"])

@@ -39,6 +39,9 @@ METRIC = "binned pair/triplet evals/s on 1 MI355X: batched FP32+guard vs plain F
 # the file lists): HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (scaled by the counters'
 # readings for a copy of known size) and VALU wave-instructions per dispatch (SQ_INSTS_VALU)
 PAIR_RMU_COUNTERS = os.path.join(ROOT, "profiles", "k_pair2_rmu_counters.json")
+# the sources the pair kernel's code object is built from: the profile records their hash
+PAIR_KERNEL_SOURCES = ["gramsci_amd/csrc/pair2.hip", "gramsci_amd/csrc/gs_device.h", "gramsci_amd/csrc/gs_common.h",
+                       "gramsci_amd/csrc/gs_kernels.h"]
 FLOPS_PER_PAIR_RMU = 22      # 3 sub, 3 mul, 2 add for s; 3 mid adds, m.s (5), |m|^2 (5), 1 product
 PEAK_FP32 = 157.3e12         # MI355X FP32 vector, dense
 PEAK_FP64 = 78.6e12
@@ -78,16 +81,27 @@ def time_gpu(eng, dxyz, dw, nd, precision, steps, warmup):
     return out, dt / steps, sum(kern_ms) / len(kern_ms), sum(grid_ms) / len(grid_ms)
 
 
+def kernel_source_hash(paths=PAIR_KERNEL_SOURCES):
+    import hashlib
+    h = hashlib.sha256()
+    for p in paths:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def pair_counters(kernel_ms_live):
-    """The pair kernel's counter readings from PAIR_RMU_COUNTERS, with 'stale' set when the kernel
-    time timed now differs from the profiled dispatch time by more than 5 %: then the readings
-    describe another build and are not used for this line."""
+    """The pair kernel's counter readings from PAIR_RMU_COUNTERS, with 'stale' set when they
+    describe another build: the kernel sources' hash differs from the one recorded with the
+    profile, or the kernel time timed now differs from the profiled dispatch time by more than
+    5 %.  Stale readings are not used for this line."""
     try:
         with open(PAIR_RMU_COUNTERS) as f:
             c = json.load(f)
     except (OSError, ValueError):
         return None
-    c["stale"] = abs(kernel_ms_live - c["kernel_ms"]) > 0.05 * c["kernel_ms"]
+    c["stale"] = (c.get("source_sha256") != kernel_source_hash()
+                  or abs(kernel_ms_live - c["kernel_ms"]) > 0.05 * c["kernel_ms"])
     c["source"] = os.path.relpath(PAIR_RMU_COUNTERS, ROOT)
     return c
 

@@ -131,6 +131,9 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const double ms = B.mu_scale;
 
     const float ua = B.ua, ub = B.ub;
+    // ub in a VGPR for the pair loop's bin estimate (an FMA reads at most one scalar operand)
+    float ubv = ub;
+    asm volatile("" : "+v"(ubv));
     const float nbm1f = (float)(nb - 1);
     const double t_lo = B.t64[0], t_hi = B.t64[nb + 1];
     const float rej_lo = B.rej_lo, rej_hi = B.rej_hi;
@@ -287,34 +290,48 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         // ---- 2. neighbour pairs
         // The triangle u < v is folded into rows: folded row r (0 <= r < K/2) holds the K pairs
         // (r, r+1+c) for c < K-1-r and (K-2-r, c) for K-1-r <= c < K (row K-2-r's pairs); for
-        // even K the last folded row is row r alone (K/2 pairs).  A row is uniform across the
-        // lanes working on it, so each pair costs a compare and two selects for its (u, v), and
-        // neighbouring lanes read neighbouring entries.
-        auto visit = [&](const int u, const int v) {
+        // even K the last folded row is row r alone (K/2 pairs).  Neighbouring lanes read
+        // neighbouring entries.
+        // The u side of a pair (its record, original index and weight) is loaded by the caller, so
+        // that a run of pairs sharing u (a row segment) loads it once and sweeps the v entries.
+        using rec_t = std::conditional_t<FP64, double4, float4>;
+        struct USide {
+            rec_t R;
+            int o;          // original index (RSD or weighted)
+            double w;       // weight * 2^shift3 (weighted)
+        };
+        auto load_u = [&](const int u) {
+            USide U;
+            if constexpr (FP64) U.R = rec64[u]; else U.R = rec32[u];
+            U.o = (nm > 1 || W) ? ol[u] : 0;
+            U.w = W ? wl[u] : 0.0;
+            return U;
+        };
+        auto visit = [&](const USide& U, const int u, const int v) {
             uint32_t pu, pv;
             int b3 = -1;
             if constexpr (FP64) {
-                const double4 Ru = rec64[u], Rv = rec64[v];
+                const double4 Ru = U.R, Rv = rec64[v];
                 pu = (uint32_t)Ru.w;
                 pv = (uint32_t)Rv.w;
                 if (!equi || ((pu ^ pv) & 0x3fffu) == 0) {
                     const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
                     if (s > t_lo && s < t_hi) {
                         const float sf = (float)s;
-                        int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(sf, LOGB, ua, ub), 0.0f, nbm1f);
+                        int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(sf, LOGB, ua, ubv), 0.0f, nbm1f);
                         while (b > 0 && s < t64[b]) --b;
                         while (b < nb && s >= t64[b + 1]) ++b;
                         if (b == nb) ++over; else b3 = b;
                     }
                 }
             } else {
-                const float4 Ru = rec32[u], Rv = rec32[v];
+                const float4 Ru = U.R, Rv = rec32[v];
                 pu = __float_as_uint(Ru.w);
                 pv = __float_as_uint(Rv.w);
                 const float dx = Rv.x - Ru.x, dy = Rv.y - Ru.y, dz = Rv.z - Ru.z;
                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                 if ((!equi || ((pu ^ pv) & 0x3fffu) == 0) && s > rej_lo && s < rej_hi) {
-                    const int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ub), 0.0f, nbm1f);
+                    const int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
                     const float2 ok = okb[b];
                     if ((s >= ok.x) & (s < ok.y)) {
                         b3 = b;
@@ -333,16 +350,17 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             // the triple product in the reference's order, (w_i w_k1) w_k2 with k1 the neighbour of
             // smaller original index (gramsci.F90:482, :501), so that sums do not depend on the
             // order in which neighbours were gathered
-            auto triple = [&](int x, int y) {
-                const bool xf = ol[x] < ol[y];
-                return (wa * ldexp(wl[xf ? x : y], -shift)) * wl[xf ? y : x];
+            auto triple = [&]() {
+                const double wv_ = wl[v];
+                const bool uf = U.o < ol[v];
+                return (wa * ldexp(uf ? U.w : wv_, -shift)) * (uf ? wv_ : U.w);
             };
             if (b3 >= 0) {
                 const int cfg = equi ? b3 : (int)tab[__umul24(pu & 0x3fffu, (uint32_t)nb) + (pv & 0x3fffu) + (uint32_t)b3];
                 int mu = 0;
                 if (nm > 1) {
                     const int mu_u = (int)((pu >> 15) & 127u), mu_v = (int)((pv >> 15) & 127u);
-                    mu = ol[u] < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
+                    mu = U.o < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
                 }
                 const int cls = ra + (int)((pu >> 14) & 1u) + (int)((pv >> 14) & 1u);
                 const int hk = (cfg * nm + mu) * 4 + cls;
@@ -350,7 +368,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     atomicAdd(&hc[(hk << rsh) + rep], 1u);
                     if constexpr (W) {
                         uint32_t qlo, qhi;
-                        quantise(triple(u, v), qlo, qhi);
+                        quantise(triple(), qlo, qhi);
                         atomicAdd(&hw[2 * hk], (unsigned long long)qlo);
                         atomicAdd(&hw[2 * hk + 1], (unsigned long long)qhi);
                     }
@@ -359,7 +377,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     atomicAdd(&a.out_cnt[go], 1ull);
                     if constexpr (W) {
                         uint32_t qlo, qhi;
-                        quantise(triple(u, v), qlo, qhi);
+                        quantise(triple(), qlo, qhi);
                         glb_add128(&a.out_w[2 * go], ((unsigned long long)qhi << 32) + qlo, 0ull);
                     }
                 }
@@ -386,12 +404,18 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const int grp = nw * G;
                 if (rr < G)
                     for (int r = wv * G + rr; r < R; r += grp) {
-                        const int t = K - 1 - r;            // pairs of row r itself
-                        const int len = (2 * r + 2 == K) ? t : K;
-                        const int u2 = K - 2 - r;
-                        for (int c = c0; c < len; c += 64) {
-                            const bool first = c < t;
-                            visit(first ? r : u2, first ? r + 1 + c : c);
+                        // the row's two segments, each with its u loaded once: (r, r+1+c) for c < t,
+                        // then (K-2-r, c) for t <= c < K (absent in an even K's last row)
+                        const int t = K - 1 - r;
+                        int c = c0;
+                        if (c < t) {
+                            const USide U = load_u(r);
+                            for (; c < t; c += 64) visit(U, r, r + 1 + c);
+                        }
+                        if (c < K && 2 * r + 2 != K) {
+                            const int u2 = K - 2 - r;
+                            const USide U = load_u(u2);
+                            for (; c < K; c += 64) visit(U, u2, c);
                         }
                     }
             } else {
@@ -400,7 +424,8 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 int r = tid / K, c = tid - r * K;
                 for (int pi = tid; pi < P; pi += nt) {
                     const bool first = c < K - 1 - r;
-                    visit(first ? r : K - 2 - r, first ? r + 1 + c : c);
+                    const int u = first ? r : K - 2 - r;
+                    visit(load_u(u), u, first ? r + 1 + c : c);
                     c += dr;
                     const bool carry = c >= K;
                     c -= carry ? K : 0;
@@ -422,12 +447,16 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 }
                 const int b1 = min(R, b0 + blk);
                 for (int r = b0 + wv; r < b1; r += nw) {
-                    const int t = K - 1 - r;                // pairs of row r itself
-                    const int len = (2 * r + 2 == K) ? t : K;
-                    const int u2 = K - 2 - r;
-                    for (int c = ln; c < len; c += 64) {
-                        const bool first = c < t;
-                        visit(first ? r : u2, first ? r + 1 + c : c);
+                    const int t = K - 1 - r;                // as in the LDS-list row schedule
+                    int c = ln;
+                    if (c < t) {
+                        const USide U = load_u(r);
+                        for (; c < t; c += 64) visit(U, r, r + 1 + c);
+                    }
+                    if (c < K && 2 * r + 2 != K) {
+                        const int u2 = K - 2 - r;
+                        const USide U = load_u(u2);
+                        for (; c < K; c += 64) visit(U, u2, c);
                     }
                 }
             }

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2(AdjArgs a)
                             // certain when the nearest mu edge (an integer 1..nm-1 of t) is farther than dt
                             const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                             safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
-                            m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
+                            m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
                             mr = nm - 1 - m;
                         }
                         if (!safe) {
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2h(AdjArgs a)
                             const float dt = fmaf(dt_ew, rq, fmaf(dt_em, rm, dt_c));
                             const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                             safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
-                            m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
+                            m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
                             mr = nm - 1 - m;
                         }
                         if (!safe) {

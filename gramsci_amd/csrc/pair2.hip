@@ -420,8 +420,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
                                     const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                                     safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
-                                    m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
-                                    mr = nm - 1 - m;
+                                    m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
                                 }
                                 bool mirrored = true;     // the FP32 mu bins are mirror images by construction
                                 if (!safe) {
@@ -447,7 +446,9 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                     } else if (W && mirror && mirrored) {
                                         put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
                                     } else {
-                                        accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mirrored, ci, cj,
+                                        // mr: the mirror image of m unless the recheck found otherwise
+                                        accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m,
+                                                           mirrored ? nm - 1 - m : mr, mirrored, ci, cj,
                                                            wis * ewj, a.out_cnt, a.out_w);
                                     }
                                 }
@@ -486,8 +487,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
                                 const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
                                 safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
-                                m = (int)__builtin_amdgcn_fmed3f(floorf(t), 0.0f, nmm1f);
-                                mr = nm - 1 - m;
+                                m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
                             }
                             if (!safe) {
                                 ++st.rechk;
@@ -504,7 +504,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             }
                             const int cj = (ef & 2) ? 0 : 1;
                             const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
-                            accumulate<RSD, false>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mirrored, ci, cj, 0.0,
+                            accumulate<RSD, false>(hc, hw, rsh, rep, nm, mirror, base, m,
+                                                   mirrored ? nm - 1 - m : mr, mirrored, ci, cj, 0.0,
                                                    a.out_cnt, a.out_w);
                         };
                         wave_sync();

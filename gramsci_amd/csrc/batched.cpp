@@ -63,7 +63,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind
     CtxView v = view(c);
     GsBatch& B = *v.batch;
     const int m = tu.m;
-    const int sx_want = kind == 2 ? (tu.sx > 0 ? tu.sx : pair_cell_sx()) : 1;
+    const int sx_want = tu.sx > 0 ? tu.sx : (kind == 2 ? pair_cell_sx() : apex_cell_sx());
     for (auto& pl : B.plans)
         if (pl.tune == tu && pl.sx == sx_want && pl.kind == kind && same_params(pl.p, p)) {
             *out = &pl;
@@ -117,7 +117,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind
     TripArgs ta{};
     int nh = 0;
     if (kind == 3) {
-        tab3 = table3(nb);
+        tab3 = table3_hist(nb);
+        if (tab3.empty()) return fail(GS_ELIMIT, "3PCF with %d bins: configuration table too large", nb);
         if (nm > 1) ntab = normal_table(nm);
         ta.ncfg = nb * (nb + 1) * (nb + 2) / 6;
         ta.nm = nm;
@@ -466,8 +467,8 @@ static int time_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kin
 
 // Launch parameters for parameter set p on this batch: the cached winner, or else the autotuner's.  The
 // tuner times the captured pipeline by coordinate descent: cells per rmax (pair 2..6, triplet 1..3),
-// then for the pair kernel the x cells per y/z cell (1, 2, 4, 8, 16), for the triplet kernel the
-// threads per work-group (64, 128, 256) and then the first-pass list length (half and twice the
+// then the x cells per y/z cell (pair 1, 2, 4, 8, 16; triplet 1, 2, 4, 8), and for the triplet kernel
+// the threads per work-group (64, 128, 256) and then the first-pass list length (half and twice the
 // automatic one).  Plans of the losing candidates are released; the winner's is kept.
 static int batch_tune(gs_ctx* c, const gs_params& p, int kind, BatchTune* out)
 {
@@ -514,6 +515,13 @@ static int batch_tune(gs_ctx* c, const gs_params& p, int kind, BatchTune* out)
         }
     } else {
         BatchTune b0 = best;
+        for (int sx : {1, 2, 4, 8}) {
+            if (sx == apex_cell_sx()) continue;     // the automatic choice, timed above
+            BatchTune t = b0;
+            t.sx = sx;
+            if ((rc = trial(t))) return rc;
+        }
+        b0 = best;
         pbest = best_plan();
         const int wg_auto = pbest ? pbest->wg1 : kWG;
         for (int wg : {64, 128, 256}) {

@@ -139,6 +139,7 @@ int pair2_shape(int nb, int nm, bool fp64, bool weighted, bool logb, int* reps, 
 double choose_h_bbox(const double* bbox, double rmax, int m, int64_t max_cells, int sx);
 int pair_cell_div(int64_t n, const double* bbox, double rmax);
 int pair_cell_sx();
+int apex_cell_sx();
 void combine2(const unsigned long long* h_out, int nb, int nm, const BinDesc& B, bool weighted, long double Wd,
               long double Wr, uint64_t* cnt, double* wsum, double* n2, double* n3, gs_stats* st);
 int check_params(const gs_params* p, int mode);
@@ -158,6 +159,7 @@ long double fx_value(unsigned long long lo, unsigned long long hi, int shift);
 int default_cell_div(const gs_ctx* c, double rmax, int mode);
 // 3PCF helpers shared by the single and batched paths (pcf34.cpp)
 std::vector<uint16_t> table3(int nb);
+std::vector<uint16_t> table3_hist(int nb);   // 4 * configuration, for the triplet kernel
 std::vector<uint8_t> normal_table(int nm);
 int plan_triplet_lds(TripArgs& a, int nb, bool fp64, bool weighted);
 // threads per triplet work-group for a mean neighbour count kmean (GS_T3_WG overrides)

@@ -42,6 +42,17 @@ std::vector<uint16_t> table3(int nb)
     return t;
 }
 
+// The triplet kernel's table: 4 * configuration (the histogram index of the configuration's class 0).
+// Empty when 4 * configurations does not fit 16 bits (then the table would not fit in LDS either).
+std::vector<uint16_t> table3_hist(int nb)
+{
+    const long ncfg = (long)nb * (nb + 1) * (nb + 2) / 6;
+    if (4 * ncfg > 65535) return {};
+    std::vector<uint16_t> t = table3(nb);
+    for (auto& x : t) x = (uint16_t)(4 * x);
+    return t;
+}
+
 // bintable for the 4PCF (gramsci.F90:1402-1420): 1-based configuration for
 // non-decreasing quadruples only, 0 elsewhere.
 static std::vector<uint16_t> table4(int nb)
@@ -238,17 +249,19 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     CtxView v = view(c);
     const bool weighted = v.has_w;
     const int m = p->cell_div > 0 ? p->cell_div : 2;
+    const int sx = apex_cell_sx();
 
     (void)hipEventRecord(v.ev[0], v.stream);
     double h;
-    if ((rc = ctx_build_grid(c, p->rmax, m, 1, &h))) return rc;
+    if ((rc = ctx_build_grid(c, p->rmax, m, sx, &h))) return rc;
     (void)hipEventRecord(v.ev[1], v.stream);
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = full_stencil(h, h, p->rmax, mst);
+    std::vector<Row> rows = full_stencil(h, h / sx, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
-    std::vector<uint16_t> tab = equi ? std::vector<uint16_t>() : table3(nb);
+    std::vector<uint16_t> tab = equi ? std::vector<uint16_t>() : table3_hist(nb);
+    if (!equi && tab.empty()) return fail(GS_ELIMIT, "3PCF with %d bins: configuration table too large", nb);
     std::vector<uint8_t> ntab = nm > 1 ? normal_table(nm) : std::vector<uint8_t>();
 
     TripArgs a{};
@@ -447,14 +460,15 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     CtxView v = view(c);
     const bool weighted = v.has_w;
     const int m = p->cell_div > 0 ? p->cell_div : 2;
+    const int sx = apex_cell_sx();
 
     (void)hipEventRecord(v.ev[0], v.stream);
     double h;
-    if ((rc = ctx_build_grid(c, p->rmax, m, 1, &h))) return rc;
+    if ((rc = ctx_build_grid(c, p->rmax, m, sx, &h))) return rc;
     (void)hipEventRecord(v.ev[1], v.stream);
     v = view(c);
     const int mst = (int)std::ceil(p->rmax / h - 1e-12);
-    std::vector<Row> rows = full_stencil(h, h, p->rmax, mst);
+    std::vector<Row> rows = full_stencil(h, h / sx, p->rmax, mst);
     BinDesc B;
     if ((rc = fill_bins(*p, 1.5 * p->rmax, ctx_xmax(c), v.wmax_d, v.wmax_r, B))) return rc;
     std::vector<uint16_t> tab = tetra ? std::vector<uint16_t>() : table4(nb);

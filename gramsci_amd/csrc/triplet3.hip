@@ -59,8 +59,9 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
     return b;
 }
 
-// record packing: bits 0..13 bin * nb (0-based bin, pre-multiplied for the configuration
-// index), bit 14 random, bits 15..21 mu bin (0-based)
+// record packing: bits 0..14 2 * bin * nb (0-based bin, pre-multiplied for the byte offset of the
+// configuration table entry), bit 15 random, bits 16..22 mu bin (0-based).  The configuration table
+// holds 4 * configuration (the histogram index of class 0).
 //
 // The variant is a template parameter so that each one pays only for its own
 // registers: MODE bit 0 RSD (nm >= 2), bit 1 equilateral (0 = plain); GEN 1 for
@@ -247,11 +248,11 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const int pos = atomicAdd(&s_k, 1);
                 if (pos >= kmax) { s_over = 1; continue; }
                 const uint32_t fj = a.flags[j];
-                uint32_t pk = __umul24((uint32_t)b, (uint32_t)nb) | ((fj & 1u) << 14);
+                uint32_t pk = __umul24((uint32_t)b, 2u * (uint32_t)nb) | ((fj & 1u) << 15);
                 if (nm > 1) {
                     int m, mr;
                     exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);   // apex -> j
-                    pk |= (uint32_t)m << 15;
+                    pk |= (uint32_t)m << 16;
                 }
                 if (nm > 1 || W) ol[pos] = a.orig[j];
                 if constexpr (FP64) {
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const double4 Ru = U.R, Rv = rec64[v];
                 pu = (uint32_t)Ru.w;
                 pv = (uint32_t)Rv.w;
-                if (!equi || ((pu ^ pv) & 0x3fffu) == 0) {
+                if (!equi || ((pu ^ pv) & 0x7fffu) == 0) {
                     const double s = exact_s(Ru.x, Ru.y, Ru.z, Rv.x, Rv.y, Rv.z);
                     if (s > t_lo && s < t_hi) {
                         const float sf = (float)s;
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 pv = __float_as_uint(Rv.w);
                 const float dx = Rv.x - Ru.x, dy = Rv.y - Ru.y, dz = Rv.z - Ru.z;
                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                if ((!equi || ((pu ^ pv) & 0x3fffu) == 0) && s > rej_lo && s < rej_hi) {
+                if ((!equi || ((pu ^ pv) & 0x7fffu) == 0) && s > rej_lo && s < rej_hi) {
                     const int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
                     const float2 ok = okb[b];
                     if ((s >= ok.x) & (s < ok.y)) {
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     }
                 }
             }
-            if (equi && b3 >= 0 && __umul24((uint32_t)b3, (uint32_t)nb) != (pu & 0x3fffu)) b3 = -1;
+            if (equi && b3 >= 0 && __umul24((uint32_t)b3, 2u * (uint32_t)nb) != (pu & 0x7fffu)) b3 = -1;
             // the triple product in the reference's order, (w_i w_k1) w_k2 with k1 the neighbour of
             // smaller original index (gramsci.F90:482, :501), so that sums do not depend on the
             // order in which neighbours were gathered
@@ -356,14 +357,18 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 return (wa * ldexp(uf ? U.w : wv_, -shift)) * (uf ? wv_ : U.w);
             };
             if (b3 >= 0) {
-                const int cfg = equi ? b3 : (int)tab[__umul24(pu & 0x3fffu, (uint32_t)nb) + (pv & 0x3fffu) + (uint32_t)b3];
+                // 4 * configuration, from the table at byte offset 2 (bin_u nb^2 + bin_v nb + b3)
+                const int cfg4 = equi ? 4 * b3
+                                      : (int)*(const uint16_t*)((const unsigned char*)tab +
+                                                                (__umul24(pu & 0x7fffu, (uint32_t)nb) + (pv & 0x7fffu) +
+                                                                 2u * (uint32_t)b3));
                 int mu = 0;
                 if (nm > 1) {
-                    const int mu_u = (int)((pu >> 15) & 127u), mu_v = (int)((pv >> 15) & 127u);
+                    const int mu_u = (int)((pu >> 16) & 127u), mu_v = (int)((pv >> 16) & 127u);
                     mu = U.o < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
                 }
-                const int cls = ra + (int)((pu >> 14) & 1u) + (int)((pv >> 14) & 1u);
-                const int hk = (cfg * nm + mu) * 4 + cls;
+                const int cls = ra + (int)((pu >> 15) & 1u) + (int)((pv >> 15) & 1u);
+                const int hk = cfg4 * nm + 4 * mu + cls;
                 if (lh) {
                     atomicAdd(&hc[(hk << rsh) + rep], 1u);
                     if constexpr (W) {

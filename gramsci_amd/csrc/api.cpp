@@ -695,16 +695,21 @@ int pair_cell_div(int64_t n, const double* bbox, double rmax)
 
 // x cells per y/z cell of the pair kernels' grid: finer x cells cut each stencil row to the x-range that
 // can reach rmax from the work item's apexes, at the cost of one more entry per cell in the cell index.
-int pair_cell_sx()
+static int cell_sx_env(const char* name, int dflt)
 {
-    const char* ev = getenv("GS_CELL_SX");      // tuning override (unset or empty: automatic)
+    const char* ev = getenv(name);                  // tuning override (unset or empty: automatic)
     if (ev && *ev) {
         int v = atoi(ev), sx = 1;
         while (sx < 64 && sx * 2 <= v) sx *= 2;
         return sx;
     }
-    return 8;
+    return dflt;
 }
+int pair_cell_sx() { return cell_sx_env("GS_CELL_SX", 8); }
+
+// x cells per y/z cell of the triplet and quad kernels' grid: their gathers scan the apex's stencil, whose
+// x-range finer x cells cut close to the sphere.
+int apex_cell_sx() { return cell_sx_env("GS_T3_CELL_SX", 1); }
 
 int default_cell_div(const gs_ctx* c, double rmax, int mode)
 {

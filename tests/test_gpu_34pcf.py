@@ -132,6 +132,23 @@ def test_3pcf_clustered_log_weighted_buffer(eng):
     check3(eng, xyz, None, 6000, O.Params(3.0, 12.0, 4, 1, False))
 
 
+@pytest.mark.parametrize("sx", ["2", "8"])
+def test_apex_x_cell_subdivisions(eng, monkeypatch, sx):
+    # the triplet and quad gathers on a grid with x cells of h / sx (GS_T3_CELL_SX): counts, sums and
+    # N2/N3 match the oracle (RSD and weighted 3PCF with buffer points, 4PCF with weights and buffers)
+    monkeypatch.setenv("GS_T3_CELL_SX", sx)
+    rng = np.random.default_rng(11)
+    d = C.clustered(5000, 110.0, 111)
+    r = C.uniform(10000, 110.0, 112)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(5000, 0.2, 113), np.ones(10000)])
+    buf = (rng.random(len(xyz)) < 0.2).astype(np.int32)
+    print(f"sx={sx}", flush=True)
+    check3(eng, xyz, w, 5000, O.Params(2.0, 13.0, 5, 3, True), buf=buf)
+    check3(eng, xyz, None, 5000, O.Params(3.0, 12.0, 4, 1, False))
+    check4(eng, xyz[:6000], w[:6000], 5000, O.Params(3.0, 8.0, 3, 1, False), buf=buf[:6000])
+
+
 @pytest.mark.parametrize("nmu", [1, 4])
 def test_3pcf_two_pass_overflow(eng, monkeypatch, nmu):
     # a first-pass neighbour list of 128 entries (about the mean neighbour count here): the

@@ -709,7 +709,7 @@ int pair_cell_sx() { return cell_sx_env("GS_CELL_SX", 8); }
 
 // x cells per y/z cell of the triplet and quad kernels' grid: their gathers scan the apex's stencil, whose
 // x-range finer x cells cut close to the sphere.
-int apex_cell_sx() { return cell_sx_env("GS_T3_CELL_SX", 1); }
+int apex_cell_sx() { return cell_sx_env("GS_T3_CELL_SX", 4); }
 
 int default_cell_div(const gs_ctx* c, double rmax, int mode)
 {

@@ -123,6 +123,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind
         ta.ncfg = nb * (nb + 1) * (nb + 2) / 6;
         ta.nm = nm;
         ta.log_bins = p.log_bins != 0;
+        ta.nrows = (int)rows.size();     // the LDS plan holds the stencil rows' table
         if ((rc = plan_triplet_lds(ta, nb, fp64, weighted))) return rc;
         nh = ta.ncfg * nm * 4;
     }
@@ -487,6 +488,10 @@ static int batch_tune(gs_ctx* c, const gs_params& p, int kind, BatchTune* out)
         float ms = 0;
         GsBatch::Plan* P = nullptr;
         int rc = time_plan(c, p, t, kind, &ms, &P);
+        if (rc == GS_ELIMIT) {
+            if (log) fprintf(stderr, "[gs tune] %dPCF m %d sx %d: %s (skipped)\n", kind, t.m, t.sx, gs_last_error());
+            return GS_OK;           // a candidate beyond a capacity limit (e.g. too many cells) is skipped
+        }
         if (rc) return rc;
         if (log)
             fprintf(stderr, "[gs tune] %dPCF m %d sx %d wg %d kmax1 %d: %.3f ms\n", kind, P->m, P->sx, P->wg1,
@@ -505,6 +510,7 @@ static int batch_tune(gs_ctx* c, const gs_params& p, int kind, BatchTune* out)
         t.m = m;
         if ((rc = trial(t))) return rc;
     }
+    if (best.m == 0) return fail(GS_ELIMIT, "batched grid too large at every tuned cell size");
     if (kind == 2) {
         const BatchTune b0 = best;
         for (int sx : {1, 2, 4, 8, 16}) {

@@ -269,6 +269,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     a.nm = nm;
     a.log_bins = p->log_bins != 0;
     a.equi = equi ? 1 : 0;
+    a.nrows = (int)rows.size();      // the LDS plan holds the stencil rows' table
     if ((rc = plan_triplet_lds(a, nb, fp64, weighted))) return rc;
 
     GsExtra& X = *v.extra;

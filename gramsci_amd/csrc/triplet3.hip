@@ -54,6 +54,7 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;           // t64
     b += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;             // config table (u16)
     if (a.nm > 1) b += ((size_t)a.nm * a.nm + 15) / 16 * 16;    // find_normal table (u8)
+    b += ((size_t)a.nrows * 12 + 15) / 16 * 16;                 // stencil rows: first point, length, offset
     // neighbour list: coords + packed bin/class/mu, sorted index, weight, orig (in LDS unless glist)
     if (!a.glist) b += kmax * triplet3_glist_entry_bytes(a, fp64, weighted);
     return b;
@@ -104,6 +105,11 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     p += ((size_t)nb * nb * nb * 2 + 15) / 16 * 16;
     uint8_t* ntab = (uint8_t*)p;
     if (nm > 1) p += ((size_t)nm * nm + 15) / 16 * 16;
+    // the apex's stencil rows: first point, number of points, and candidate number of the first point
+    int32_t* rw_js = (int32_t*)p;
+    int32_t* rw_len = rw_js + a.nrows;
+    int32_t* rw_off = rw_len + a.nrows;
+    p += ((size_t)a.nrows * 12 + 15) / 16 * 16;
     // the neighbour list: in LDS, or (GEN 2) in this work-group's part of device memory
     if constexpr (GEN == 2) p = (unsigned char*)a.glist + (size_t)blockIdx.x * (size_t)a.glist_stride;
     float4* rec32 = (float4*)p;          // FP32: x, y, z relative to the apex, packed bin/class/mu
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     if (W) p += (size_t)kmax * 8;
     int32_t* ol = (int32_t*)p;           // RSD or weighted: original index of each neighbour
 
-    __shared__ int s_item, s_k, s_over;
+    __shared__ int s_item, s_k, s_over, s_ncand;
 
     if (lh) {
         for (int k = tid; k < nh * reps; k += nt) hc[k] = 0;
@@ -217,24 +223,50 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         const int cz = min(max((int)floor((Pa.z - lz) * inv_h), 0), nz - 1);
 
         const unsigned long long cand0 = cand, over0 = over;   // restored if the apex is queued
-        // ---- 1. gather the kept neighbours with their exact bin.  The candidates of all
-        // stencil rows are numbered consecutively and thread t takes numbers t, t + nt, ...,
-        // so every thread is busy even where a row holds fewer points than the work-group
-        // has threads (coarse cells at low density).
-        int flat = 0;           // candidates in the rows before this one (work-group uniform)
-        int next = tid;         // this thread's next candidate number
-        for (int r = 0; r < a.nrows; ++r) {
+        // ---- 1. gather the kept neighbours with their exact bin.  The stencil rows' point ranges are
+        // read in parallel (one row per thread) into LDS and numbered by a prefix sum over the rows, so
+        // the gather waits for one round of index loads, not one per row; the candidates of all rows
+        // are then numbered consecutively and thread t takes numbers t, t + nt, ..., so every thread is
+        // busy even where a row holds fewer points than the work-group has threads.
+        for (int r = tid; r < a.nrows; r += nt) {
             const Row rw = a.rows[r];
             const int by = cy + rw.dy, bz = cz + rw.dz;
-            if (by < 0 || by >= ny || bz < 0 || bz >= nz) continue;
             const int bx0 = max(cx + rw.x0, 0), bx1 = min(cx + rw.x1, nx - 1);
-            if (bx0 > bx1) continue;
-            const int rb = cb + (bz * ny + by) * nx;
-            const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
-            const int row_end = flat + (je - js);
-            const int j0 = js - flat;
-            flat = row_end;
-            for (; next < row_end; next += nt) {
+            int js = 0, len = 0;
+            if (by >= 0 && by < ny && bz >= 0 && bz < nz && bx0 <= bx1) {
+                const int rb = cb + (bz * ny + by) * nx;
+                js = a.cell_start[rb + bx0];
+                len = a.cell_start[rb + bx1 + 1] - js;
+            }
+            rw_js[r] = js;
+            rw_len[r] = len;
+        }
+        __syncthreads();
+        if (tid < 64) {         // exclusive prefix of the row lengths, 64 rows at a time (wave 0)
+            int run = 0;
+            for (int b0 = 0; b0 < a.nrows; b0 += 64) {
+                const int x = b0 + tid < a.nrows ? rw_len[b0 + tid] : 0;
+                int inc = x;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int y = __shfl_up(inc, off, 64);
+                    if (tid >= off) inc += y;
+                }
+                if (b0 + tid < a.nrows) rw_off[b0 + tid] = run + inc - x;
+                run += __shfl(inc, 63, 64);
+            }
+            if (tid == 0) s_ncand = run;
+        }
+        __syncthreads();
+        const int ncand = s_ncand;
+        int row = 0, row_beg = 0, row_end = a.nrows > 0 ? rw_len[0] : 0, j0 = a.nrows > 0 ? rw_js[0] : 0;
+        {
+            for (int next = tid; next < ncand; next += nt) {
+                while (next >= row_end) {           // the row holding candidate number next
+                    ++row;
+                    row_beg = rw_off[row];
+                    row_end = row_beg + rw_len[row];
+                    j0 = rw_js[row] - row_beg;
+                }
                 const int j = j0 + next;
                 ++cand;
                 const double4 Pj = a.pw[j];

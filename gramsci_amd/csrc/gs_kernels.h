@@ -160,7 +160,7 @@ struct QuadArgs {
 // bytes per neighbour of a device-memory neighbour list (glist) of the quad kernel: x, y, z, w,
 // key, permutation, packed bin, and the sort's staging copy of x, y, z, w and the packed bin
 constexpr size_t kQuadGlistEntryBytes = 4 * 8 + 3 * 4 + 4 * 8 + 4;
-size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool glist = false);
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra, int nrows, bool glist = false);
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st);
 int quad4_occupancy(const QuadArgs& a, int nb, bool weighted);
 

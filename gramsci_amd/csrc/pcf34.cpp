@@ -481,13 +481,13 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     const size_t budget = 160 * 1024;
     int kmax = 256;
     bool lds_hist = (size_t)ncell * 64 <= 16 * 1024;
-    while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) kmax /= 2;
-    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) {
+    while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra, (int)rows.size()) > budget) kmax /= 2;
+    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra, (int)rows.size()) > budget) {
         lds_hist = false;
         kmax = 256;
-        while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget) kmax /= 2;
+        while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra, (int)rows.size()) > budget) kmax /= 2;
     }
-    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra) > budget)
+    if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra, (int)rows.size()) > budget)
         return fail(GS_ELIMIT, "4PCF tables do not fit in LDS");
     {
         const char* ev = getenv("GS_T4_KMAX_LDS");  // test override: a shorter second-pass list

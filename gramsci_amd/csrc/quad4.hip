@@ -31,12 +31,13 @@ namespace gs {
 
 using namespace dev;
 
-size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra, bool glist)
+size_t quad4_lds_bytes(int nb, int ncell, int kmax, bool lds_hist, bool tetra, int nrows, bool glist)
 {
     size_t b = 0;
     if (lds_hist) b += (size_t)ncell * 16 * 4;
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;            // t64
     if (!tetra) b += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;   // perimeter table (u16)
+    b += ((size_t)nrows * 12 + 15) / 16 * 16;                    // stencil rows: first point, length, offset
     if (glist) return b;                                         // neighbour list in device memory
     b += (size_t)kmax * (24 + 8 + 4 + 4 + 4);                     // x, y, z, w, key, slot/packed, sorted j
     b += ((size_t)kmax * kmax + 15) / 16 * 16;                    // link matrix (u8: bin+1, 0 = not kept)
@@ -65,6 +66,11 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
     uint16_t* tab = (uint16_t*)p;
     if (!tetra) p += ((size_t)nb * nb * nb * nb * 2 + 15) / 16 * 16;
+    // the apex's stencil rows: first point, number of points, and candidate number of the first point
+    int32_t* rw_js = (int32_t*)p;
+    int32_t* rw_len = rw_js + a.nrows;
+    int32_t* rw_off = rw_len + a.nrows;
+    p += ((size_t)a.nrows * 12 + 15) / 16 * 16;
     if constexpr (BIG) p = (unsigned char*)a.glist + (size_t)blockIdx.x * (size_t)a.glist_stride;
     double* nx_ = (double*)p;
     double* ny_ = nx_ + kmax;
@@ -75,7 +81,7 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
     int32_t* jl = pk + kmax;                 // bin | rand << 8 in gather order
     uint8_t* L = (uint8_t*)(jl + kmax);      // L[u * kmax + v] (LDS list; BIG: the sort's staging area)
 
-    __shared__ int s_item, s_k, s_over;
+    __shared__ int s_item, s_k, s_over, s_ncand;
 
     if (lh)
         for (int k = tid; k < ncell * 16; k += kWG) hc[k] = 0;
@@ -128,22 +134,49 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
         const int cz = min(max((int)floor((Pa.z - a.g.lo[2]) * inv_h), 0), nzg - 1);
 
         const unsigned long long cand0 = cand, over0 = over;   // restored if the apex is queued
-        // ---- 1. gather: the candidates of all stencil rows are numbered consecutively and
-        // thread t takes numbers t, t + 256, ... (every thread busy even where rows are short)
-        int flat = 0;           // candidates in the rows before this one (work-group uniform)
-        int next = tid;         // this thread's next candidate number
-        for (int r = 0; r < a.nrows; ++r) {
+        // ---- 1. gather: the stencil rows' point ranges are read in parallel (one row per thread) into
+        // LDS and numbered by a prefix sum over the rows; the candidates of all rows are then numbered
+        // consecutively and thread t takes numbers t, t + 256, ... (every thread busy even where rows
+        // are short)
+        for (int r = tid; r < a.nrows; r += kWG) {
             const Row rw = a.rows[r];
             const int by = cy + rw.dy, bz = cz + rw.dz;
-            if (by < 0 || by >= nyg || bz < 0 || bz >= nzg) continue;
             const int bx0 = max(cx + rw.x0, 0), bx1 = min(cx + rw.x1, nxg - 1);
-            if (bx0 > bx1) continue;
-            const int rb = (bz * nyg + by) * nxg;
-            const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
-            const int row_end = flat + (je - js);
-            const int j0 = js - flat;
-            flat = row_end;
-            for (; next < row_end; next += kWG) {
+            int js = 0, len = 0;
+            if (by >= 0 && by < nyg && bz >= 0 && bz < nzg && bx0 <= bx1) {
+                const int rb = (bz * nyg + by) * nxg;
+                js = a.cell_start[rb + bx0];
+                len = a.cell_start[rb + bx1 + 1] - js;
+            }
+            rw_js[r] = js;
+            rw_len[r] = len;
+        }
+        __syncthreads();
+        if (tid < 64) {         // exclusive prefix of the row lengths, 64 rows at a time (wave 0)
+            int run = 0;
+            for (int b0 = 0; b0 < a.nrows; b0 += 64) {
+                const int x = b0 + tid < a.nrows ? rw_len[b0 + tid] : 0;
+                int inc = x;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int y = __shfl_up(inc, off, 64);
+                    if (tid >= off) inc += y;
+                }
+                if (b0 + tid < a.nrows) rw_off[b0 + tid] = run + inc - x;
+                run += __shfl(inc, 63, 64);
+            }
+            if (tid == 0) s_ncand = run;
+        }
+        __syncthreads();
+        const int ncand = s_ncand;
+        int row = 0, row_end = a.nrows > 0 ? rw_len[0] : 0, j0 = a.nrows > 0 ? rw_js[0] : 0;
+        {
+            for (int next = tid; next < ncand; next += kWG) {
+                while (next >= row_end) {           // the row holding candidate number next
+                    ++row;
+                    const int row_beg = rw_off[row];
+                    row_end = row_beg + rw_len[row];
+                    j0 = rw_js[row] - row_beg;
+                }
                 const int j = j0 + next;
                 ++cand;
                 const double4 Pj = a.pw[j];
@@ -354,7 +387,7 @@ static hipError_t launch_q(const QuadArgs& a, size_t lds, int nblocks, hipStream
 hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, hipStream_t st)
 {
     const bool big = a.glist != nullptr;
-    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0, big);
+    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0, a.nrows, big);
     if (big) return weighted ? launch_q<true, true>(a, lds, nblocks, st) : launch_q<false, true>(a, lds, nblocks, st);
     return weighted ? launch_q<true, false>(a, lds, nblocks, st) : launch_q<false, false>(a, lds, nblocks, st);
 }
@@ -362,7 +395,7 @@ hipError_t launch_quad4(const QuadArgs& a, int nb, bool weighted, int nblocks, h
 int quad4_occupancy(const QuadArgs& a, int nb, bool weighted)
 {
     const bool big = a.glist != nullptr;
-    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0, big);
+    const size_t lds = quad4_lds_bytes(nb, a.ncell, a.kmax, a.lds_hist != 0, a.tetra != 0, a.nrows, big);
     void (*kern)(QuadArgs) = big ? (weighted ? k_quad4<true, true> : k_quad4<false, true>)
                                  : (weighted ? k_quad4<true, false> : k_quad4<false, false>);
     if (lds > 64 * 1024 &&

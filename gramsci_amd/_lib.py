@@ -35,7 +35,7 @@ class gs_params(ctypes.Structure):
 class gs_stats(ctypes.Structure):
     _fields_ = [("ms_grid", ctypes.c_double), ("ms_count", ctypes.c_double), ("ms_kernel_avg", ctypes.c_double),
                 ("n_units", ctypes.c_uint64), ("n_binned", ctypes.c_uint64), ("n_candidates", ctypes.c_uint64),
-                ("n_recheck", ctypes.c_uint64), ("n_overflow", ctypes.c_uint64),
+                ("n_recheck", ctypes.c_uint64), ("n_overflow", ctypes.c_uint64), ("n_evaluated", ctypes.c_uint64),
                 ("cell_div", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
     def as_dict(self):

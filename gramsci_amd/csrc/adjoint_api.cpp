@@ -146,6 +146,7 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
         st->ms_count = ms_count;
         st->ms_kernel_avg = ms_count;
         st->n_candidates = hs[0];
+        st->n_evaluated = hs[0];
         st->n_recheck = hs[1];
         st->cell_div = mst;
     }

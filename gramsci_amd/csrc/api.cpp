@@ -801,6 +801,7 @@ void combine2(const unsigned long long* h_out, int nb, int nm, const BinDesc& B,
         st->n_units = total;
         st->n_binned = total;
         st->n_candidates = h_out[3 * nbt + 0];
+        st->n_evaluated = st->n_candidates;
         st->n_recheck = h_out[3 * nbt + 1];
         st->n_overflow = h_out[3 * nbt + 2];
     }

@@ -622,6 +622,7 @@ extern "C" int gs_count_2pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         st->n_binned = total;
         const unsigned long long* s = h.data() + (size_t)nreg * 3 * nbt;
         st->n_candidates = s[0];
+        st->n_evaluated = s[0];
         st->n_recheck = s[1];
         st->n_overflow = s[2];
         st->cell_div = P->mst;
@@ -679,6 +680,7 @@ extern "C" int gs_count_3pcf_batched(gs_ctx* c, const gs_params* p, uint64_t* cn
         st->n_recheck = s[1];
         st->n_overflow = s[2];
         st->n_units = s[3];
+        st->n_evaluated = s[4];
         st->n_binned = 0;           // the binned triplets are the counts' total
         for (size_t k = 0; k < (size_t)nreg * nh; ++k) st->n_binned += h[k];
         st->cell_div = P->mst;

@@ -120,7 +120,8 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted);
 hipError_t launch_triplet3(const TripArgs& a, int nb, bool fp64, bool weighted, int nblocks, hipStream_t st);
 int triplet3_occupancy(const TripArgs& a, int nb, bool fp64, bool weighted);
 // neighbour counts of the apexes t * stride (t < nsample) of a single catalogue; -1 for buffer points
-hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int32_t* out, hipStream_t st);
+hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int after_only, int32_t* out,
+                            hipStream_t st);
 
 // Arguments of the persistent 4PCF quad kernel (quad4.hip).
 struct QuadArgs {

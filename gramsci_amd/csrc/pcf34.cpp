@@ -81,6 +81,7 @@ static void fill_stats(gs_stats* st, hipEvent_t* ev, const unsigned long long* s
     st->n_overflow = s[2];
     st->n_units = s[3];
     st->n_binned = s[4];
+    st->n_evaluated = s[3];
     st->cell_div = m;
 }
 
@@ -331,7 +332,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     if (nsample > 0) {
         int32_t* d_cnt = X.i32.ptr + 2 * n + 3;
         std::vector<int32_t> kc(nsample);
-        e = launch_nbr_count(a, stride, nsample, d_cnt, v.stream);
+        e = launch_nbr_count(a, stride, nsample, nm > 1 ? 0 : 1, d_cnt, v.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(kc.data(), d_cnt, nsample * sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
         if (e != hipSuccess) return hip_fail(e, "3PCF neighbour sample");
@@ -422,6 +423,7 @@ static int count3(gs_ctx* c, const gs_params* p, bool equi, uint64_t* cnt, doubl
     combine3(S.data(), ncfg * nm, v.wsum_d, v.wsum_r, !(equi && nm > 1), n2, n3);
     fill_stats(st, v.ev, h_out.data() + 3 * nh, mst);
     if (st) {
+        st->n_evaluated = h_out[3 * nh + 4];
         st->n_binned = 0;           // the binned triplets are the counts' total
         for (int k = 0; k < nh; ++k) st->n_binned += h_out[k];
     }
@@ -561,7 +563,7 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
         ts.nrows = a.nrows;
         int32_t* d_cnt = X.i32.ptr + 2 * n + 3;
         std::vector<int32_t> kc(nsample);
-        e = launch_nbr_count(ts, stride, nsample, d_cnt, v.stream);
+        e = launch_nbr_count(ts, stride, nsample, 0, d_cnt, v.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(kc.data(), d_cnt, nsample * sizeof(int32_t), hipMemcpyDeviceToHost, v.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(v.stream);
         if (e != hipSuccess) return hip_fail(e, "4PCF neighbour sample");

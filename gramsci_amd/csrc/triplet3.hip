@@ -83,6 +83,12 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     const int kmax = a.kmax;
     const bool lh = !GEN || a.lds_hist != 0;
     constexpr bool equi = (MODE & kT3Equi) != 0;
+    // Isotropic modes visit each triangle once: a triangle's configuration (its sorted side bins) and
+    // class (its randoms) do not depend on which vertex is the apex, so the reference's count at each
+    // vertex that is not a buffer point is added at once by the first vertex in the order "points that
+    // are not buffer points, by index, then buffer points", whose list keeps only the neighbours after
+    // it.  (The RSD mode bins by the apex's mu values and visits every apex's pairs.)
+    constexpr bool SYM = (MODE & kT3Rsd) == 0;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;              // threads per work-group: 256, or 64 for short neighbour lists
     const int lane = tid & 63;
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     if (W) p += (size_t)kmax * 8;
     int32_t* ol = (int32_t*)p;           // RSD or weighted: original index of each neighbour
 
-    __shared__ int s_item, s_k, s_over, s_ncand;
+    __shared__ int s_item, s_k, s_over, s_ncand, s_kall;
 
     if (lh) {
         for (int k = tid; k < nh * reps; k += nt) hc[k] = 0;
@@ -170,6 +176,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     };
 
     unsigned long long units = 0, rechk = 0, over = 0, cand = 0;   // binned triplets: the histogram's total
+    unsigned long long evals = 0;   // neighbour pairs whose third side this kernel evaluated
     unsigned long long bound = 0;   // u32 increments since the last drain (uniform)
     const int xcd = blockIdx.x & 7;
     int tried = 0;
@@ -183,6 +190,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             s_item = b_next;        // -1 at the end
             if (b_next >= 0) ++b_next;
             s_k = 0;
+            s_kall = 0;
             s_over = 0;
         }
         __syncthreads();
@@ -277,10 +285,17 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 while (b > 0 && s < t64[b]) --b;
                 while (b < nb && s >= t64[b + 1]) ++b;
                 if (b == nb) { ++over; continue; }
+                const uint32_t fj = a.flags[j];
+                if constexpr (SYM) {
+                    // every kept neighbour counts for the apex's C(k, 2) candidates; only those after the
+                    // apex in the visiting order (buffer points, then larger index) enter its list
+                    atomicAdd(&s_kall, 1);
+                    if (!((fj & 2u) || j > apex)) continue;
+                }
                 const int pos = atomicAdd(&s_k, 1);
                 if (pos >= kmax) { s_over = 1; continue; }
-                const uint32_t fj = a.flags[j];
                 uint32_t pk = __umul24((uint32_t)b, 2u * (uint32_t)nb) | ((fj & 1u) << 15);
+                if constexpr (SYM) pk |= (fj & 2u) << 15;       // bit 16: buffer point (never an apex)
                 if (nm > 1) {
                     int m, mr;
                     exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);   // apex -> j
@@ -316,7 +331,12 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         // pair index: K <= 4096 in LDS, so P < 2^31; a device-memory list may hold more
         using pidx_t = std::conditional_t<GEN == 2, long long, int>;
         const pidx_t P = (pidx_t)K * (K - 1) / 2;
-        if (tid == 0) units += (unsigned long long)P;
+        if (tid == 0) {
+            const pidx_t kall = SYM ? (pidx_t)s_kall : (pidx_t)K;
+            units += (unsigned long long)(kall * (kall - 1) / 2);   // the reference's apex-pair candidates
+            evals += (unsigned long long)P;
+        }
+        const int oa = SYM && W ? a.orig[apex] : 0;     // the apex's original index
         const int ra = (int)(fa & 1u);
         const double wa = Pa.w;
 
@@ -401,22 +421,38 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 }
                 const int cls = ra + (int)((pu >> 15) & 1u) + (int)((pv >> 15) & 1u);
                 const int hk = cfg4 * nm + 4 * mu + cls;
+                // the reference's count at each vertex that is not a buffer point (the apex always)
+                const bool bu = SYM && ((pu >> 16) & 1u), bv = SYM && ((pv >> 16) & 1u);
+                const uint32_t mult = SYM ? 3u - (uint32_t)bu - (uint32_t)bv : 1u;
+                unsigned long long qs = 0;      // fixed-point sum of those vertices' triple products
+                if constexpr (W) {
+                    auto q64 = [](double x) {
+                        uint32_t lo, hi;
+                        quantise(x, lo, hi);
+                        return ((unsigned long long)hi << 32) + lo;
+                    };
+                    qs = q64(triple());
+                    if constexpr (SYM) {
+                        // vertex p's product in the reference's order, (w_p w_x) w_y with x the other
+                        // vertex of smaller original index; scaled weights W = w 2^shift, as for the apex
+                        const double Wa = ldexp(wa, shift), Wv = wl[v];
+                        const int ov = ol[v], ou = U.o;
+                        if (!bu) qs += q64(oa < ov ? (ldexp(U.w, -shift) * wa) * Wv
+                                                   : (ldexp(U.w, -shift) * ldexp(Wv, -shift)) * Wa);
+                        if (!bv) qs += q64(oa < ou ? (ldexp(Wv, -shift) * wa) * U.w
+                                                   : (ldexp(Wv, -shift) * ldexp(U.w, -shift)) * Wa);
+                    }
+                }
                 if (lh) {
-                    atomicAdd(&hc[(hk << rsh) + rep], 1u);
+                    atomicAdd(&hc[(hk << rsh) + rep], mult);
                     if constexpr (W) {
-                        uint32_t qlo, qhi;
-                        quantise(triple(), qlo, qhi);
-                        atomicAdd(&hw[2 * hk], (unsigned long long)qlo);
-                        atomicAdd(&hw[2 * hk + 1], (unsigned long long)qhi);
+                        atomicAdd(&hw[2 * hk], qs & 0xffffffffull);
+                        atomicAdd(&hw[2 * hk + 1], qs >> 32);
                     }
                 } else {
                     const size_t go = (size_t)cur_r * nh + hk;
-                    atomicAdd(&a.out_cnt[go], 1ull);
-                    if constexpr (W) {
-                        uint32_t qlo, qhi;
-                        quantise(triple(), qlo, qhi);
-                        glb_add128(&a.out_w[2 * go], ((unsigned long long)qhi << 32) + qlo, 0ull);
-                    }
+                    atomicAdd(&a.out_cnt[go], (unsigned long long)mult);
+                    if constexpr (W) glb_add128(&a.out_w[2 * go], qs, 0ull);
                 }
             }
         };
@@ -498,7 +534,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 }
             }
         }
-        bound += GEN == 2 ? (1ull << 31) + 1 : (unsigned long long)P;   // LDS increments since the last drain
+        bound += GEN == 2 ? (1ull << 31) + 1 : (unsigned long long)P * (SYM ? 3 : 1);   // LDS increments since the last drain
         __syncthreads();
         if (lh && bound > (1ull << 31)) {
             drain(cur_r);
@@ -510,6 +546,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
     if (lh && cur_r >= 0) drain(cur_r);
     for (int off = 32; off > 0; off >>= 1) {
         units += __shfl_down(units, off, 64);
+        evals += __shfl_down(evals, off, 64);
         rechk += __shfl_down(rechk, off, 64);
         over += __shfl_down(over, off, 64);
         cand += __shfl_down(cand, off, 64);
@@ -519,6 +556,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
         if (rechk) atomicAdd(&a.out_stats[1], rechk);
         if (over) atomicAdd(&a.out_stats[2], over);
         if (units) atomicAdd(&a.out_stats[3], units);
+        if (evals) atomicAdd(&a.out_stats[4], evals);
     }
 }
 
@@ -565,7 +603,10 @@ static hipError_t launch_t3(T3Kernel kern, const TripArgs& a, size_t lds, int nb
 // Neighbour counts (rmin < d < rmax, exact FP64 distances) of the sampled apexes
 // t * stride, t < nsample, over the full stencil of the catalogue's grid; -1 for a
 // buffer point.  Sizes the first pass's neighbour list (one thread per apex).
-__global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int nsample, int32_t* __restrict__ out)
+// after_only: count only the neighbours after the apex in the isotropic 3PCF's visiting order (buffer
+// points, or a larger index), which are the ones its list holds
+__global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int nsample, int after_only,
+                                                   int32_t* __restrict__ out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsample) return;
@@ -594,16 +635,17 @@ __global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int n
         for (int j = js; j < je; ++j) {
             const double4 Pj = a.pw[j];
             const double s = exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z);
-            cnt += (s > t_lo && s < t_hi) ? 1 : 0;
+            cnt += (s > t_lo && s < t_hi && (!after_only || j > apex || (a.flags[j] & 2u))) ? 1 : 0;
         }
     }
     out[t] = cnt;
 }
 
-hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int32_t* out, hipStream_t st)
+hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int after_only, int32_t* out, hipStream_t st)
 {
     if (nsample <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nbr_count, dim3((nsample + 255) / 256), dim3(256), 0, st, a, stride, nsample, out);
+    hipLaunchKernelGGL(k_nbr_count, dim3((nsample + 255) / 256), dim3(256), 0, st, a, stride, nsample, after_only,
+                       out);
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ module gramsci_hip_api
 
   type, bind(C) :: gs_stats
      real(c_double) :: ms_grid, ms_count, ms_kernel_avg
-     integer(c_int64_t) :: n_units, n_binned, n_candidates, n_recheck, n_overflow
+     integer(c_int64_t) :: n_units, n_binned, n_candidates, n_recheck, n_overflow, n_evaluated
      integer(c_int32_t) :: cell_div, reserved
   end type gs_stats
 

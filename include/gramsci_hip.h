@@ -74,6 +74,9 @@ typedef struct gs_stats {
     uint64_t n_recheck;      /* FP64 re-evaluations triggered by the guard band */
     uint64_t n_overflow;     /* pairs whose reference bin is nbins+1 (not counted; the reference
                                 writes them out of bounds) */
+    uint64_t n_evaluated;    /* evaluations the kernel performed: candidate pairs tested (2PCF),
+                                neighbour pairs whose third side was tested (3PCF; a quarter of
+                                n_units or less where each triangle is visited once), quads (4PCF) */
     int32_t  cell_div;       /* cells per rmax actually used */
     int32_t  reserved;
 } gs_stats;

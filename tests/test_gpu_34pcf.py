@@ -2,9 +2,10 @@
 reference's output files.
 
 Bar: triplet / quad counts per configuration and class bit-exact (both
-precisions for the 3PCF; the 4PCF is evaluated in FP64); N2/N3 (and the disc
-sums) within 1e-12 relative of the oracle, with an absolute floor of
-1e-12 * max|N| for cells where a signed sum cancels.
+precisions for the 3PCF; the 4PCF is evaluated in FP64); N2/N3 within 1e-12
+relative of the oracle, with an absolute floor of 1e-12 * max|N| for cells
+where a signed sum cancels; the 4PCF disc sums (FP64 atomic sums, whose order
+varies) within 1e-12 relative with a floor of 1e-9 * max|disc|.
 """
 import numpy as np
 import pytest
@@ -44,6 +45,17 @@ def _close(a, b, rel=REL):
     return np.all(np.abs(a[fin] - b[fin]) <= np.maximum(rel * np.abs(b[fin]), floor))
 
 
+def _close_disc(a, b):
+    # the 4PCF disc sums are FP64 atomic sums (their order varies from run to run) of signed terms; a
+    # cell whose terms cancel carries their rounding, about n * 2^-53 of their size, so the floor is
+    # 1e-9 of the largest cell
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    fin = np.isfinite(b)
+    floor = 1e-9 * max(np.abs(b[fin]).max() if fin.any() else 0.0, 1e-300)
+    return np.all(np.abs(a[fin] - b[fin]) <= np.maximum(REL * np.abs(b[fin]), floor))
+
+
 def check3(eng, xyz, w, nd, p, buf=None, precisions=("fp64", "fp32")):
     ref = O.pcf3(xyz, w, nd, p, buf=buf)
     outs = []
@@ -54,6 +66,9 @@ def check3(eng, xyz, w, nd, p, buf=None, precisions=("fp64", "fp32")):
               f"recheck={r['stats']['n_recheck']}", flush=True)
         assert np.array_equal(r["cnt"], ref["cnt"]), (prec, np.argwhere(r["cnt"] != ref["cnt"])[:5])
         assert r["stats"]["n_units"] == ref["n_cand"]
+        # RSD visits every apex's pairs; the isotropic modes each triangle once (shorter lists)
+        ev = r["stats"]["n_evaluated"]
+        assert ev == ref["n_cand"] if p.nmu > 1 else ev <= ref["n_cand"]
         assert _close(r["n2"], ref["n2"]) and _close(r["n3"], ref["n3"])
         outs.append(r)
     return ref, outs
@@ -67,7 +82,7 @@ def check4(eng, xyz, w, nd, p, buf=None):
     assert np.array_equal(r["cnt"], ref["cnt"])
     assert r["stats"]["n_units"] == ref["n_quads"]
     assert _close(r["n2"], ref["n2"]) and _close(r["n3"], ref["n3"])
-    assert _close(r["disc_n"], ref["disc_n"]) and _close(r["disc_r"], ref["disc_r"])
+    assert _close_disc(r["disc_n"], ref["disc_n"]) and _close_disc(r["disc_r"], ref["disc_r"])
     return ref, r
 
 
@@ -326,7 +341,7 @@ def test_4pcf_two_pass_overflow(eng, monkeypatch, tetra):
     if not tetra:
         ref = O.pcf4(xyz, w, 3000, p)
         assert np.array_equal(two["cnt"], ref["cnt"]) and two["stats"]["n_units"] == ref["n_quads"]
-        assert _close(two["disc_n"], ref["disc_n"]) and _close(two["disc_r"], ref["disc_r"])
+        assert _close_disc(two["disc_n"], ref["disc_n"]) and _close_disc(two["disc_r"], ref["disc_r"])
 
 
 def test_4pcf_lattice_and_log(eng):
@@ -452,11 +467,11 @@ def test_4pcf_device_memory_neighbour_lists(eng, monkeypatch, tetra, weighted):
     assert np.array_equal(out["cnt"], base["cnt"])
     for k in ("n_units", "n_binned", "n_candidates", "n_overflow"):
         assert out["stats"][k] == base["stats"][k], k
-    assert _close(out["n2"], base["n2"]) and _close(out["disc_n"], base["disc_n"])
+    assert _close(out["n2"], base["n2"]) and _close_disc(out["disc_n"], base["disc_n"])
     ref = (O.pcf4_tetra if tetra else O.pcf4)(xyz, w, 3000, p, buf=buf)
     assert np.array_equal(out["cnt"], ref["cnt"])
     assert _close(out["n2"], ref["n2"]) and _close(out["n3"], ref["n3"])
-    assert _close(out["disc_n"], ref["disc_n"]) and _close(out["disc_r"], ref["disc_r"])
+    assert _close_disc(out["disc_n"], ref["disc_n"]) and _close_disc(out["disc_r"], ref["disc_r"])
 
 
 @pytest.mark.parametrize("tetra", [False, True])

@@ -2,8 +2,9 @@
 the HIP path against the oracle and the reference's output files.
 
 Bar: counts per bin (and mu bin) and class bit-exact in both precisions
-(the tetrahedron kernel is FP64); N2/N3/disc sums within 1e-12 relative of
-the oracle, with an absolute floor of 1e-12 * max|N| for cancelling cells.
+(the tetrahedron kernel is FP64); N2/N3 within 1e-12 relative of the oracle,
+with an absolute floor of 1e-12 * max|N| for cancelling cells; the disc sums
+(FP64 atomic sums, whose order varies) with a floor of 1e-9 * max|disc|.
 """
 import numpy as np
 import pytest
@@ -43,6 +44,17 @@ def _close(a, b, rel=REL):
     return np.all(np.abs(a[fin] - b[fin]) <= np.maximum(rel * np.abs(b[fin]), floor))
 
 
+def _close_disc(a, b):
+    # the 4PCF disc sums are FP64 atomic sums (their order varies from run to run) of signed terms; a
+    # cell whose terms cancel carries their rounding, about n * 2^-53 of their size, so the floor is
+    # 1e-9 of the largest cell
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    fin = np.isfinite(b)
+    floor = 1e-9 * max(np.abs(b[fin]).max() if fin.any() else 0.0, 1e-300)
+    return np.all(np.abs(a[fin] - b[fin]) <= np.maximum(REL * np.abs(b[fin]), floor))
+
+
 def check_equi(eng, xyz, w, nd, p, buf=None):
     ref = O.pcf3_equi(xyz, w, nd, p, buf=buf)
     outs = []
@@ -63,7 +75,7 @@ def check_tetra(eng, xyz, w, nd, p, buf=None):
     print(f"  tetra: binned={r['stats']['n_binned']}", flush=True)
     assert np.array_equal(r["cnt"], ref["cnt"])
     assert _close(r["n2"], ref["n2"]) and _close(r["n3"], ref["n3"])
-    assert _close(r["disc_n"], ref["disc_n"]) and _close(r["disc_r"], ref["disc_r"])
+    assert _close_disc(r["disc_n"], ref["disc_n"]) and _close_disc(r["disc_r"], ref["disc_r"])
     return ref, r
 
 

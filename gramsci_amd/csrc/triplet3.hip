@@ -603,16 +603,19 @@ static hipError_t launch_t3(T3Kernel kern, const TripArgs& a, size_t lds, int nb
 // Neighbour counts (rmin < d < rmax, exact FP64 distances) of the sampled apexes
 // t * stride, t < nsample, over the full stencil of the catalogue's grid; -1 for a
 // buffer point.  Sizes the first pass's neighbour list (one thread per apex).
+// Kept-neighbour counts of sampled apexes (every stride-th point; -1 for a buffer point), for sizing the
+// first pass's lists: one wave per sampled apex, its lanes over the apex's stencil candidates.
 // after_only: count only the neighbours after the apex in the isotropic 3PCF's visiting order (buffer
-// points, or a larger index), which are the ones its list holds
+// points, or a larger index), which are the ones its list holds.
 __global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int nsample, int after_only,
                                                    int32_t* __restrict__ out)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);   // wave-uniform
+    const int lane = threadIdx.x & 63;
     if (t >= nsample) return;
     const int apex = t * stride;
     if (a.flags[apex] & 2u) {
-        out[t] = -1;
+        if (lane == 0) out[t] = -1;
         return;
     }
     const BinDesc& B = *a.bins;
@@ -632,20 +635,22 @@ __global__ __launch_bounds__(256) void k_nbr_count(TripArgs a, int stride, int n
         if (bx0 > bx1) continue;
         const int rb = (bz * ny + by) * nx;
         const int js = a.cell_start[rb + bx0], je = a.cell_start[rb + bx1 + 1];
-        for (int j = js; j < je; ++j) {
+        for (int j = js + lane; j < je; j += 64) {
             const double4 Pj = a.pw[j];
             const double s = exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z);
             cnt += (s > t_lo && s < t_hi && (!after_only || j > apex || (a.flags[j] & 2u))) ? 1 : 0;
         }
     }
-    out[t] = cnt;
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+    if (lane == 0) out[t] = cnt;
 }
 
 hipError_t launch_nbr_count(const TripArgs& a, int stride, int nsample, int after_only, int32_t* out, hipStream_t st)
 {
     if (nsample <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nbr_count, dim3((nsample + 255) / 256), dim3(256), 0, st, a, stride, nsample, after_only,
-                       out);
+    const long long threads = (long long)nsample * 64;
+    hipLaunchKernelGGL(k_nbr_count, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, a, stride, nsample,
+                       after_only, out);
     return hipGetLastError();
 }
 

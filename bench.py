@@ -378,7 +378,9 @@ def bench_c3(args):
     leg = triplet_leg(args.steps, args.warmup, not args.no_fp64)
     line = line_base("apex-pair candidates/s, 3PCF (config 3)", "triplet evals/s", leg["value"], args.steps,
                      args.warmup, leg["ms_per_step"] * 1e-3, "f32", leg["workload"],
-                     units_per_step=leg["units_per_step"], binned_triplets=leg["binned_triplets"])
+                     units_per_step=leg["units_per_step"],
+                     evaluated_pairs_per_step=leg["evaluated_pairs_per_step"],
+                     binned_triplets=leg["binned_triplets"])
     line["roofline"] = dict(leg["roofline"], kernel_ms=leg["kernel_ms"], traffic=None)
     if "fp64" in leg:
         line["fp64"] = leg["fp64"]

@@ -225,19 +225,20 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     // The run's sum is one 64-bit integer: products are below 2^52 (fx_sd, fx_sr) and a run ends
     // after at most 33 increments, so it cannot overflow; the flush splits it at bit 32 into
     // the two LDS halves, which keeps value = hi * 2^32 + lo.  The update is written with
-    // selects, not branches, so that no register copies are needed where paths merge.
-    int ck = -1;
+    // selects, not branches, so that no register copies are needed where paths merge.  An empty
+    // run (cc = 0) stays attached to its last bin (bin 0 at the start): flushing it adds zeros, so
+    // the flush needs no test for a missing run.
+    int ck = 0;
     uint32_t cc = 0;
     unsigned long long cq = 0;
     auto flush = [&]() {
-        if (ck >= 0) {
-            atomicAdd(&hc[(ck << rsh) + rep], cc);
-            if constexpr (W) {
-                atomicAdd(&hw[2 * ck], cq & 0xffffffffull);
-                atomicAdd(&hw[2 * ck + 1], cq >> 32);
-            }
-            ck = -1;
+        atomicAdd(&hc[(ck << rsh) + rep], cc);
+        if constexpr (W) {
+            atomicAdd(&hw[2 * ck], cq & 0xffffffffull);
+            atomicAdd(&hw[2 * ck + 1], cq >> 32);
         }
+        cc = 0;
+        cq = 0;
     };
     auto put = [&](int k, uint32_t c, unsigned long long q) {   // c in {1, 2}
         const bool nw = (k != ck) || (cc >= 32u);
@@ -392,12 +393,15 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             e = a.pl[jl];
                             if constexpr (W) ewj = ldexp(a.pw[jl].w, (ef & 1u) ? sr : sd);
                         }
-                        // entry index and flags in one rotated register: ep = eidx << 2 | flags (-1: no
-                        // entry), so the self-row test eidx >= jmin is ep >= jmin << 2
-                        int ep = has ? (le << 2) | (int)(ef & 3u) : -1;
-                        const int jmin4 = jmin << 2;
-                        for (int k = 0; k < (skip ? 0 : P); ++k) {
-                            if (vi && ep >= jmin4) do {
+                        // entry index, class and buffer flag in one rotated register (-1: no entry):
+                        // ep = eidx << 16 | (class offset of a random entry, nb*nm, or 0) << 1 | buffer,
+                        // so the self-row test eidx >= jmin is ep >= jmin << 16 and the pair's class
+                        // offset is cb0 + a field of ep (nb*nm <= 127*127 fits in 15 bits)
+                        int ep = has ? (le << 16) | ((ef & 1u) ? nbm << 1 : 0) | (int)((ef >> 1) & 1u) : -1;
+                        const int jmin16 = jmin << 16;
+                        const int kend = __builtin_amdgcn_readfirstlane(skip ? 0 : P);
+                        for (int k = 0; k < kend; ++k) {
+                            if (vi && ep >= jmin16) do {
                                 const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
                                 const float dy = e.y - yi_r, dz = e.z - zi_r;
                                 const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
                                 const float2 ok = okb[b];
                                 bool safe = (s >= ok.x) && (s < ok.y);
-                                int m = 0, mr = 0;
+                                int m = 0, mr;      // mr is read only after an FP64 recheck has set it
                                 if constexpr (RSD) {
                                     const float r = __builtin_amdgcn_rsqf(s);
                                     // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
@@ -425,20 +429,20 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 bool mirrored = true;     // the FP32 mu bins are mirror images by construction
                                 if (!safe) {
                                     ++st.rechk;
-                                    const double4 Pj = a.pw[jt + (ep >> 2)];
+                                    const double4 Pj = a.pw[jt + (ep >> 16)];
                                     Pi = a.pw[i];
                                     const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
                                     if (!(se > t_lo && se < t_hi)) break;
                                     b = exact_bin(se, t64, nb);
-                                    if (b == nb) { st.over += (uint64_t)(ci + ((ep & 2) ? 0 : 1)); break; }
+                                    if (b == nb) { st.over += (uint64_t)(ci + (ep & 1 ? 0 : 1)); break; }
                                     if constexpr (RSD) {
                                         exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
                                         mirrored = mr == nm - 1 - m;
                                     }
                                 }
                                 {
-                                    const int cj = (ep & 2) ? 0 : 1;
-                                    const int base = ((ep & 1) ? cb1 : cb0) + __umul24(b, nm);
+                                    const int cj = (ep & 1) ? 0 : 1;
+                                    const int base = cb0 + ((ep >> 1) & 0x7fff) + __umul24(b, nm);
                                     unsigned long long q = 0;
                                     if constexpr (W) q = quantise_rn(wis * ewj);
                                     if constexpr (W && !RSD) {

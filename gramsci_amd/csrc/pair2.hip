@@ -34,7 +34,8 @@
 // are added explicitly.
 //
 // Histograms live in LDS, shared by the work-group's waves: u32 counts
-// replicated `reps` times (replica = lane % reps) to spread same-bin atomics,
+// replicated `reps` times (replica = lane % reps, one plane of nbt counts per
+// replica) to spread same-bin atomics,
 // and per bin two u64 sums of the weight product's fixed-point value split at
 // bit 32 (no carries, so no returning atomics).  A wave that has issued 2^29
 // increments drains the histogram with atomic exchanges into the global
@@ -53,11 +54,11 @@ struct Stats {
 };
 
 // One pair's histogram update.  base = class offset + b*nm (flat bin index
-// without the mu bin); counts live at ((base + m) << rsh) + rep.  mirrored:
+// without the mu bin); counts live at hcr[base + m], hcr = this lane's replica plane.  mirrored:
 // mr == nm-1-m (always so for the FP32 mu bins; tested after an FP64 recheck).
 // Returns the number of LDS bin increments made.
 template <bool RSD, bool W>
-__device__ __forceinline__ int accumulate(uint32_t* hc, unsigned long long* hw, int rsh, int rep, int nm,
+__device__ __forceinline__ int accumulate(uint32_t* hcr, unsigned long long* hw, int nm,
                                           bool mirror, int base, int m, int mr, bool mirrored, int ci, int cj,
                                           double x, unsigned long long* out_cnt, unsigned long long* out_w)
 {
@@ -68,7 +69,7 @@ __device__ __forceinline__ int accumulate(uint32_t* hc, unsigned long long* hw, 
         qhi = (uint32_t)(q >> 32);
     }
     auto add = [&](int k, uint32_t c) {
-        atomicAdd(&hc[(k << rsh) + rep], c);
+        atomicAdd(&hcr[k], c);
         if constexpr (W) {
             atomicAdd(&hw[2 * k], (unsigned long long)qlo * c);
             atomicAdd(&hw[2 * k + 1], (unsigned long long)qhi * c);
@@ -108,7 +109,7 @@ __device__ void drain(uint32_t* hc, unsigned long long* hw, int nbt, int nm, int
     for (int k = t0; k < nbt; k += stride) {
         const int km = mirror ? (k - k % nm) + (nm - 1 - k % nm) : k;
         unsigned long long s = 0;
-        for (int r = 0; r < reps; ++r) s += atomicExch(&hc[k * reps + r], 0u);
+        for (int r = 0; r < reps; ++r) s += atomicExch(&hc[k + r * nbt], 0u);
         if (s) {
             atomicAdd(&out_cnt[k], s);
             if (mirror) atomicAdd(&out_cnt[km], s);
@@ -159,7 +160,6 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     const int lane = tid & (kWave - 1);
     const int wv = tid / kWave;
     const int rep = lane & (reps - 1);
-    const int rsh = 31 - __clz(reps);
     const bool mirror = RSD && a.mirror;
     const int nthr = blockDim.x;        // a multiple of 64, at most 1024
     const int copies = batched ? nthr / kWave : 1;
@@ -172,6 +172,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     // the histogram this wave adds to: shared by the work-group, or wave-private in batched mode
     uint32_t* hc = batched ? hc0 + (size_t)wv * nbt : hc0;
     unsigned long long* hw = batched ? hw0 + (size_t)wv * 2 * nbt : hw0;
+    uint32_t* hcr = hc + rep * nbt;      // this lane's replica plane of the counts
     float2* okb = (float2*)p;
     p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     uint32_t cc = 0;
     unsigned long long cq = 0;
     auto flush = [&]() {
-        atomicAdd(&hc[(ck << rsh) + rep], cc);
+        atomicAdd(&hcr[ck], cc);
         if constexpr (W) {
             atomicAdd(&hw[2 * ck], cq & 0xffffffffull);
             atomicAdd(&hw[2 * ck + 1], cq >> 32);
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                     const int base = ((fj & 1) ? cb1 : cb0) + __umul24(b, nm);
                     double x = 0.0;
                     if constexpr (W) x = wis * wj;
-                    accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m, mr, mr == nm - 1 - m, ci, cj, x,
+                    accumulate<RSD, W>(hcr, hw, nm, mirror, base, m, mr, mr == nm - 1 - m, ci, cj, x,
                                        a.out_cnt, a.out_w);
                 };
                 if constexpr (FP64) {
@@ -451,7 +452,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                         put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
                                     } else {
                                         // mr: the mirror image of m unless the recheck found otherwise
-                                        accumulate<RSD, W>(hc, hw, rsh, rep, nm, mirror, base, m,
+                                        accumulate<RSD, W>(hcr, hw, nm, mirror, base, m,
                                                            mirrored ? nm - 1 - m : mr, mirrored, ci, cj,
                                                            wis * ewj, a.out_cnt, a.out_w);
                                     }
@@ -500,34 +501,36 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
                                 if (!(se > t_lo && se < t_hi)) return;
                                 b = exact_bin(se, t64, nb);
-                                if (b == nb) { st.over += (uint64_t)(ci + ((ef & 2) ? 0 : 1)); return; }
+                                if (b == nb) { st.over += (uint64_t)(ci + ((ef & 1) ? 0 : 1)); return; }
                                 if constexpr (RSD) {
                                     exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
                                     mirrored = mr == nm - 1 - m;
                                 }
                             }
-                            const int cj = (ef & 2) ? 0 : 1;
-                            const int base = ((ef & 1) ? cb1 : cb0) + __umul24(b, nm);
-                            accumulate<RSD, false>(hc, hw, rsh, rep, nm, mirror, base, m,
+                            const int cj = (ef & 1) ? 0 : 1;
+                            const int base = cb0 + (int)(ef >> 1) + __umul24(b, nm);
+                            accumulate<RSD, false>(hcr, hw, nm, mirror, base, m,
                                                    mirrored ? nm - 1 - m : mr, mirrored, ci, cj, 0.0,
                                                    a.out_cnt, a.out_w);
                         };
+                        // the tile's flags word holds the entry's class offset (nb*nm for a random, 0 for a
+                        // data point) shifted left by one, and its buffer bit in bit 0
                         wave_sync();
                         uint32_t fl = 2u;
                         if (lane < nt) {
                             tl[lane] = a.pl[jt + lane];
                             fl = a.flags[jt + lane];
-                            tlf[lane] = fl;
+                            tlf[lane] = ((fl & 1u) ? (uint32_t)nbm << 1 : 0u) | ((fl >> 1) & 1u);
                         }
                         const bool skip = chunk_buf && __ballot((fl & 2u) == 0) == 0;
                         wave_sync();
                         const int kend = (vi && !skip) ? nt : 0;
-                        const int lm = lane % nt;
+                        int kk = lane % nt;        // this lane's entry at step k: (k + lane) mod nt
                         for (int k = 0; k < kend; ++k) {
-                            int kk = k + lm;
-                            kk = kk >= nt ? kk - nt : kk;
-                            if (kk < jmin) continue;
-                            eval32(tl[kk], tlf[kk], kk);
+                            const int kc = kk;
+                            kk = kk + 1 == nt ? 0 : kk + 1;
+                            if (kc < jmin) continue;
+                            eval32(tl[kc], tlf[kc], kc);
                         }
                     }
                 }

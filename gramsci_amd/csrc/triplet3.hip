@@ -360,10 +360,69 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             U.w = W ? wl[u] : 0.0;
             return U;
         };
+        // histogram update of the triangle (apex, u, v) whose third side has bin b3
+        auto update = [&](const USide& U, const int v, const uint32_t pu, const uint32_t pv, const int b3) {
+            if (equi && __umul24((uint32_t)b3, 2u * (uint32_t)nb) != (pu & 0x7fffu)) return;
+            // the triple product in the reference's order, (w_i w_k1) w_k2 with k1 the neighbour of
+            // smaller original index (gramsci.F90:482, :501), so that sums do not depend on the
+            // order in which neighbours were gathered
+            auto triple = [&]() {
+                const double wv_ = wl[v];
+                const bool uf = U.o < ol[v];
+                return (wa * ldexp(uf ? U.w : wv_, -shift)) * (uf ? wv_ : U.w);
+            };
+            // 4 * configuration, from the table at byte offset 2 (bin_u nb^2 + bin_v nb + b3)
+            const int cfg4 = equi ? 4 * b3
+                                  : (int)*(const uint16_t*)((const unsigned char*)tab +
+                                                            (__umul24(pu & 0x7fffu, (uint32_t)nb) + (pv & 0x7fffu) +
+                                                             2u * (uint32_t)b3));
+            int mu = 0;
+            if (nm > 1) {
+                const int mu_u = (int)((pu >> 16) & 127u), mu_v = (int)((pv >> 16) & 127u);
+                mu = U.o < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
+            }
+            const int cls = ra + (int)((pu >> 15) & 1u) + (int)((pv >> 15) & 1u);
+            const int hk = cfg4 * nm + 4 * mu + cls;
+            // the reference's count at each vertex that is not a buffer point (the apex always)
+            const bool bu = SYM && ((pu >> 16) & 1u), bv = SYM && ((pv >> 16) & 1u);
+            const uint32_t mult = SYM ? 3u - (uint32_t)bu - (uint32_t)bv : 1u;
+            unsigned long long qs = 0;      // fixed-point sum of those vertices' triple products
+            if constexpr (W) {
+                auto q64 = [](double x) {
+                    uint32_t lo, hi;
+                    quantise(x, lo, hi);
+                    return ((unsigned long long)hi << 32) + lo;
+                };
+                qs = q64(triple());
+                if constexpr (SYM) {
+                    // vertex p's product in the reference's order, (w_p w_x) w_y with x the other
+                    // vertex of smaller original index; scaled weights W = w 2^shift, as for the apex
+                    const double Wa = ldexp(wa, shift), Wv = wl[v];
+                    const int ov = ol[v], ou = U.o;
+                    if (!bu) qs += q64(oa < ov ? (ldexp(U.w, -shift) * wa) * Wv
+                                               : (ldexp(U.w, -shift) * ldexp(Wv, -shift)) * Wa);
+                    if (!bv) qs += q64(oa < ou ? (ldexp(Wv, -shift) * wa) * U.w
+                                               : (ldexp(Wv, -shift) * ldexp(U.w, -shift)) * Wa);
+                }
+            }
+            if (lh) {
+                atomicAdd(&hc[(hk << rsh) + rep], mult);
+                if constexpr (W) {
+                    atomicAdd(&hw[2 * hk], qs & 0xffffffffull);
+                    atomicAdd(&hw[2 * hk + 1], qs >> 32);
+                }
+            } else {
+                const size_t go = (size_t)cur_r * nh + hk;
+                atomicAdd(&a.out_cnt[go], (unsigned long long)mult);
+                if constexpr (W) glb_add128(&a.out_w[2 * go], qs, 0ull);
+            }
+        };
+        // the FP32 path calls update where the bin is known (the certain interval, or after the FP64
+        // recheck), so no "no bin" sentinel is kept and tested at a join
         auto visit = [&](const USide& U, const int u, const int v) {
             uint32_t pu, pv;
-            int b3 = -1;
             if constexpr (FP64) {
+                int b3 = -1;
                 const double4 Ru = U.R, Rv = rec64[v];
                 pu = (uint32_t)Ru.w;
                 pv = (uint32_t)Rv.w;
@@ -377,6 +436,7 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                         if (b == nb) ++over; else b3 = b;
                     }
                 }
+                if (b3 >= 0) update(U, v, pu, pv, b3);
             } else {
                 const float4 Ru = U.R, Rv = rec32[v];
                 pu = __float_as_uint(Ru.w);
@@ -387,72 +447,16 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                     const int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
                     const float2 ok = okb[b];
                     if ((s >= ok.x) & (s < ok.y)) {
-                        b3 = b;
+                        update(U, v, pu, pv, b);
                     } else {
                         ++rechk;
                         const double4 Pu = a.pw[jl[u]], Pv = a.pw[jl[v]];
                         const double se = exact_s(Pu.x, Pu.y, Pu.z, Pv.x, Pv.y, Pv.z);
                         if (se > t_lo && se < t_hi) {
                             const int be = exact_bin(se, t64, nb);
-                            if (be == nb) ++over; else b3 = be;
+                            if (be == nb) ++over; else update(U, v, pu, pv, be);
                         }
                     }
-                }
-            }
-            if (equi && b3 >= 0 && __umul24((uint32_t)b3, 2u * (uint32_t)nb) != (pu & 0x7fffu)) b3 = -1;
-            // the triple product in the reference's order, (w_i w_k1) w_k2 with k1 the neighbour of
-            // smaller original index (gramsci.F90:482, :501), so that sums do not depend on the
-            // order in which neighbours were gathered
-            auto triple = [&]() {
-                const double wv_ = wl[v];
-                const bool uf = U.o < ol[v];
-                return (wa * ldexp(uf ? U.w : wv_, -shift)) * (uf ? wv_ : U.w);
-            };
-            if (b3 >= 0) {
-                // 4 * configuration, from the table at byte offset 2 (bin_u nb^2 + bin_v nb + b3)
-                const int cfg4 = equi ? 4 * b3
-                                      : (int)*(const uint16_t*)((const unsigned char*)tab +
-                                                                (__umul24(pu & 0x7fffu, (uint32_t)nb) + (pv & 0x7fffu) +
-                                                                 2u * (uint32_t)b3));
-                int mu = 0;
-                if (nm > 1) {
-                    const int mu_u = (int)((pu >> 16) & 127u), mu_v = (int)((pv >> 16) & 127u);
-                    mu = U.o < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
-                }
-                const int cls = ra + (int)((pu >> 15) & 1u) + (int)((pv >> 15) & 1u);
-                const int hk = cfg4 * nm + 4 * mu + cls;
-                // the reference's count at each vertex that is not a buffer point (the apex always)
-                const bool bu = SYM && ((pu >> 16) & 1u), bv = SYM && ((pv >> 16) & 1u);
-                const uint32_t mult = SYM ? 3u - (uint32_t)bu - (uint32_t)bv : 1u;
-                unsigned long long qs = 0;      // fixed-point sum of those vertices' triple products
-                if constexpr (W) {
-                    auto q64 = [](double x) {
-                        uint32_t lo, hi;
-                        quantise(x, lo, hi);
-                        return ((unsigned long long)hi << 32) + lo;
-                    };
-                    qs = q64(triple());
-                    if constexpr (SYM) {
-                        // vertex p's product in the reference's order, (w_p w_x) w_y with x the other
-                        // vertex of smaller original index; scaled weights W = w 2^shift, as for the apex
-                        const double Wa = ldexp(wa, shift), Wv = wl[v];
-                        const int ov = ol[v], ou = U.o;
-                        if (!bu) qs += q64(oa < ov ? (ldexp(U.w, -shift) * wa) * Wv
-                                                   : (ldexp(U.w, -shift) * ldexp(Wv, -shift)) * Wa);
-                        if (!bv) qs += q64(oa < ou ? (ldexp(Wv, -shift) * wa) * U.w
-                                                   : (ldexp(Wv, -shift) * ldexp(U.w, -shift)) * Wa);
-                    }
-                }
-                if (lh) {
-                    atomicAdd(&hc[(hk << rsh) + rep], mult);
-                    if constexpr (W) {
-                        atomicAdd(&hw[2 * hk], qs & 0xffffffffull);
-                        atomicAdd(&hw[2 * hk + 1], qs >> 32);
-                    }
-                } else {
-                    const size_t go = (size_t)cur_r * nh + hk;
-                    atomicAdd(&a.out_cnt[go], (unsigned long long)mult);
-                    if constexpr (W) glb_add128(&a.out_w[2 * go], qs, 0ull);
                 }
             }
         };

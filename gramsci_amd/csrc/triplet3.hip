@@ -61,8 +61,9 @@ size_t triplet3_lds_bytes(const TripArgs& a, int nb, bool fp64, bool weighted)
 }
 
 // record packing: bits 0..14 2 * bin * nb (0-based bin, pre-multiplied for the byte offset of the
-// configuration table entry), bit 15 random, bits 16..22 mu bin (0-based).  The configuration table
-// holds 4 * configuration (the histogram index of class 0).
+// configuration table entry), byte 2 the mu bin (0-based, RSD) or the buffer flag (0/1, otherwise),
+// byte 3 the class (1: random).  The configuration table holds 4 * configuration (the histogram index
+// of class 0).
 //
 // The variant is a template parameter so that each one pays only for its own
 // registers: MODE bit 0 RSD (nm >= 2), bit 1 equilateral (0 = plain); GEN 1 for
@@ -294,8 +295,10 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 }
                 const int pos = atomicAdd(&s_k, 1);
                 if (pos >= kmax) { s_over = 1; continue; }
-                uint32_t pk = __umul24((uint32_t)b, 2u * (uint32_t)nb) | ((fj & 1u) << 15);
-                if constexpr (SYM) pk |= (fj & 2u) << 15;       // bit 16: buffer point (never an apex)
+                // whole bytes for the flags let the pair loop add and subtract them with byte selects
+                // (SDWA) instead of extracting bits
+                uint32_t pk = __umul24((uint32_t)b, 2u * (uint32_t)nb) | ((fj & 1u) << 24);
+                if constexpr (SYM) pk |= (fj & 2u) << 15;       // byte 2 = 1: buffer point (never an apex)
                 if (nm > 1) {
                     int m, mr;
                     exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);   // apex -> j
@@ -352,12 +355,15 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
             rec_t R;
             int o;          // original index (RSD or weighted)
             double w;       // weight * 2^shift3 (weighted)
+            uint32_t m3;    // SYM: 3 - (u is a buffer point), the multiplicity before v's flag
         };
         auto load_u = [&](const int u) {
             USide U;
             if constexpr (FP64) U.R = rec64[u]; else U.R = rec32[u];
             U.o = (nm > 1 || W) ? ol[u] : 0;
             U.w = W ? wl[u] : 0.0;
+            if constexpr (FP64) U.m3 = 3u - (((uint32_t)U.R.w >> 16) & 0xffu);
+            else U.m3 = 3u - ((__float_as_uint(U.R.w) >> 16) & 0xffu);
             return U;
         };
         // histogram update of the triangle (apex, u, v) whose third side has bin b3
@@ -381,11 +387,11 @@ __global__ __launch_bounds__(kWG) void k_triplet3(TripArgs a)
                 const int mu_u = (int)((pu >> 16) & 127u), mu_v = (int)((pv >> 16) & 127u);
                 mu = U.o < ol[v] ? ntab[mu_u * nm + mu_v] : ntab[mu_v * nm + mu_u];
             }
-            const int cls = ra + (int)((pu >> 15) & 1u) + (int)((pv >> 15) & 1u);
+            const int cls = ra + (int)(pu >> 24) + (int)(pv >> 24);
             const int hk = cfg4 * nm + 4 * mu + cls;
             // the reference's count at each vertex that is not a buffer point (the apex always)
-            const bool bu = SYM && ((pu >> 16) & 1u), bv = SYM && ((pv >> 16) & 1u);
-            const uint32_t mult = SYM ? 3u - (uint32_t)bu - (uint32_t)bv : 1u;
+            const uint32_t bu = SYM ? (pu >> 16) & 0xffu : 0u, bv = SYM ? (pv >> 16) & 0xffu : 0u;
+            const uint32_t mult = SYM ? U.m3 - bv : 1u;
             unsigned long long qs = 0;      // fixed-point sum of those vertices' triple products
             if constexpr (W) {
                 auto q64 = [](double x) {

@@ -326,7 +326,8 @@ __global__ __launch_bounds__(kWG) void k_adjoint2h(AdjArgs a)
                 int ep = has ? (le << 2) | (int)(ef & 3u) : -1;   // entry index << 2 | flags, -1: none
                 const int jmin4 = jmin << 2;
                 long long qi = 0, qj = 0;
-                for (int k = 0; k < (skip ? 0 : P); ++k) {
+                const int kend = __builtin_amdgcn_readfirstlane(skip ? 0 : P);     // wave-uniform
+                for (int k = 0; k < kend; ++k) {
                     if (vi && ep >= jmin4) do {
                         const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
                         const float dy = e.y - yi_r, dz = e.z - zi_r;

@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint2h(AdjArgs a)
                 double ewj = 0.0;
                 if (!skip) {
                     e = a.pl[jl];
-                    const double wj = a.pw[jl].w;
+                    const double wj = a.ws[jl];
                     ewj = wj * ((ef & 1u) ? wsr : wsd);
                 }
                 int ep = has ? (le << 2) | (int)(ef & 3u) : -1;   // entry index << 2 | flags, -1: none

@@ -91,6 +91,7 @@ static int adjoint_impl(gs_ctx* c, const gs_params* p, const double* gN2, const 
 
     AdjArgs a{};
     a.pw = v.pw;
+    a.ws = v.ws;
     a.pl = v.pl;
     a.flags = v.flags;
     a.orig = v.orig;

@@ -338,6 +338,7 @@ struct gs_ctx {
     int grid_sx = 1;
     DevBuf<int32_t> cell_of, cell_start, fill, chunk_nc, chunk_off, scan_tmp;
     DevBuf<double4> pw;
+    DevBuf<double> ws;          // sorted raw weights
     DevBuf<float4> pl;
     DevBuf<uint32_t> flags;
     DevBuf<int32_t> orig;
@@ -398,7 +399,7 @@ extern "C" int gs_destroy(gs_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     c->xyz.release(); c->w.release(); c->buf.release();
     c->cell_of.release(); c->cell_start.release(); c->fill.release(); c->chunk_nc.release();
-    c->chunk_off.release(); c->scan_tmp.release(); c->pw.release(); c->pl.release(); c->flags.release();
+    c->chunk_off.release(); c->scan_tmp.release(); c->pw.release(); c->ws.release(); c->pl.release(); c->flags.release();
     c->orig.release();
     c->params.release(); c->out.release(); c->queue.release(); c->part.release(); c->wpart.release();
     extra_release(c->extra);
@@ -637,7 +638,8 @@ int grid_build_raw(const double* xyz, const double* w, const int32_t* buf, int64
     if (e == hipSuccess) e = exclusive_scan(b.fill, b.cell_start, ncell + 1, b.scan_tmp, st);
     if (e == hipSuccess) e = launch_fill_u32(b.fill, ncell + 1, 0u, st);
     if (e == hipSuccess)
-        e = launch_scatter(xyz, w, buf, n, n_data, g, b.cell_of, b.cell_start, b.fill, b.pw, b.pl, b.flags, b.orig, st);
+        e = launch_scatter(xyz, w, buf, n, n_data, g, b.cell_of, b.cell_start, b.fill, b.pw, b.ws, b.pl, b.flags, b.orig,
+                           st);
     const int nrow = g.dim[1] * g.dim[2];
     if (e == hipSuccess) e = launch_row_chunks(b.cell_start, g.dim[0], nrow, b.chunk_nc, st);
     if (e == hipSuccess) e = exclusive_scan(b.chunk_nc, b.chunk_off, nrow + 1, b.scan_tmp, st);
@@ -660,11 +662,11 @@ int build_grid(gs_ctx* c, double h, int sx)
     int rc;
     if ((rc = c->cell_of.reserve(n)) || (rc = c->cell_start.reserve(ncell + 1)) || (rc = c->fill.reserve(ncell + 1)) ||
         (rc = c->chunk_nc.reserve(ncell + 1)) || (rc = c->chunk_off.reserve(ncell + 1)) ||
-        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) || (rc = c->pl.reserve(n)) ||
-        (rc = c->flags.reserve(n)) || (rc = c->orig.reserve(n)))
+        (rc = c->scan_tmp.reserve(scan_scratch_size(ncell + 1))) || (rc = c->pw.reserve(n)) || (rc = c->ws.reserve(n)) ||
+        (rc = c->pl.reserve(n)) || (rc = c->flags.reserve(n)) || (rc = c->orig.reserve(n)))
         return rc;
     GridBufs b{c->cell_of.ptr, c->cell_start.ptr, c->fill.ptr, c->chunk_nc.ptr, c->chunk_off.ptr, c->scan_tmp.ptr,
-               c->pw.ptr, c->pl.ptr, c->flags.ptr, c->orig.ptr};
+               c->pw.ptr, c->ws.ptr, c->pl.ptr, c->flags.ptr, c->orig.ptr};
     GridDesc g;
     int32_t nch = 0;
     if ((rc = grid_build_raw(c->xyz.ptr, c->has_w ? c->w.ptr : nullptr, c->has_buf ? c->buf.ptr : nullptr, n,
@@ -854,6 +856,7 @@ extern "C" int gs_count_2pcf(gs_ctx* c, const gs_params* p, uint64_t* cnt, doubl
 
     PairArgs a{};
     a.pw = c->pw.ptr;
+    a.ws = c->ws.ptr;
     a.pl = c->pl.ptr;
     a.flags = c->flags.ptr;
     a.cell_start = c->cell_start.ptr;
@@ -962,6 +965,7 @@ CtxView view(gs_ctx* c)
     for (int k = 0; k < 6; ++k) v.bbox[k] = c->bbox[k];
     v.g = c->g;
     v.pw = c->pw.ptr;
+    v.ws = c->ws.ptr;
     v.flags = c->flags.ptr;
     v.orig = c->orig.ptr;
     v.cell_start = c->cell_start.ptr;

@@ -37,7 +37,8 @@ void batch_release(GsBatch& b)
     b.plans.clear();
     b.d_off.release(); b.d_row_base.release(); b.cell_of.release(); b.count.release(); b.cell_start.release();
     b.nch.release(); b.row_chunk.release(); b.scan_tmp.release(); b.queue.release(); b.orig.release();
-    b.d_regions.release(); b.params.release(); b.pw.release(); b.pl.release(); b.flags.release(); b.out.release();
+    b.d_regions.release(); b.params.release(); b.pw.release(); b.ws.release(); b.pl.release(); b.flags.release();
+    b.out.release();
     b.xyz.release(); b.w.release(); b.buf.release(); b.ovf.release(); b.big.release();
     b.valid = false;
 }
@@ -147,7 +148,8 @@ static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind
         (rc = B.nch.reserve(rows_tot + 1)) || (rc = B.row_chunk.reserve(rows_tot + 1)) ||
         (rc = B.scan_tmp.reserve(scan_scratch_size(scan_n))) || (rc = B.queue.reserve(32)) ||
         (kind == 3 && (rc = B.ovf.reserve(2 * n + 3))) ||
-        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.pl.reserve(n)) || (rc = B.flags.reserve(n)) ||
+        (rc = B.orig.reserve(n)) || (rc = B.pw.reserve(n)) || (rc = B.ws.reserve(n)) || (rc = B.pl.reserve(n)) ||
+        (rc = B.flags.reserve(n)) ||
         (rc = B.out.reserve(nout)))
         return rc;
     // per-plan parameter blob: regions, bins, stencil rows (kept for the plan's lifetime)
@@ -180,6 +182,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind
 
     PairArgs a{};
     a.pw = B.pw.ptr;
+    a.ws = B.ws.ptr;
     a.pl = B.pl.ptr;
     a.flags = B.flags.ptr;
     a.cell_start = B.cell_start.ptr;
@@ -291,7 +294,7 @@ static int get_plan(gs_ctx* c, const gs_params& p, const BatchTune& tu, int kind
     if (e == hipSuccess)
         e = launch_scatter_b(B.xyz.ptr, B.has_w ? B.w.ptr : nullptr, B.has_buf ? B.buf.ptr : nullptr, n, B.d_off.ptr,
                              nreg, d_reg, h, hx, B.cell_of.ptr, B.cell_start.ptr,
-                             B.count.ptr, B.pw.ptr, B.pl.ptr, B.flags.ptr, B.orig.ptr, v.stream);
+                             B.count.ptr, B.pw.ptr, B.ws.ptr, B.pl.ptr, B.flags.ptr, B.orig.ptr, v.stream);
     if (kind == 2) {
         if (e == hipSuccess)
             e = launch_row_chunks_b(B.cell_start.ptr, d_reg, a.row_base, nreg, (int)rows_tot, B.nch.ptr, v.stream);

@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz,
                                                  const int32_t* __restrict__ buf, int64_t n, int64_t n_data, GridDesc g,
                                                  const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
                                                  int32_t* __restrict__ fill, double4* __restrict__ pw,
-                                                 float4* __restrict__ pl, uint32_t* __restrict__ flags,
-                                                 int32_t* __restrict__ orig, int pass)
+                                                 double* __restrict__ ws, float4* __restrict__ pl,
+                                                 uint32_t* __restrict__ flags, int32_t* __restrict__ orig, int pass)
 {
     const int nx = g.dim[0], ny = g.dim[1];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -210,7 +210,9 @@ __global__ __launch_bounds__(256) void k_scatter(const double* __restrict__ xyz,
         const int c = cell_of[i];
         const int pos = start[c] + atomicAdd(&fill[c], 1);
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-        pw[pos] = make_double4(x, y, z, w ? w[i] : 1.0);
+        const double wi = w ? w[i] : 1.0;
+        pw[pos] = make_double4(x, y, z, wi);
+        ws[pos] = wi;
         const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
         // cell-local FP32 coordinates; .w carries the cell's x index (exact in FP32)
         pl[pos] = make_float4((float)(x - (g.lo[0] + cx * g.hx)), (float)(y - (g.lo[1] + cy * g.h)),
@@ -273,7 +275,7 @@ hipError_t launch_cell_count(const double* xyz, int64_t n, const GridDesc& g, in
 
 hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
                           const GridDesc& g, const int32_t* cell_of, const int32_t* start, int32_t* fill,
-                          double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
+                          double4* pw, double* ws, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
 {
     int nblk = (int)((n + 255) / 256);
     if (nblk > 65535) nblk = 65535;
@@ -281,7 +283,7 @@ hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf
     // two passes on one stream: a cell's data points take its first slots, its randoms the rest
     for (int pass = 0; pass < 2; ++pass)
         hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(256), 0, st, xyz, w, buf, n, n_data, g, cell_of, start, fill,
-                           pw, pl, flags, orig, pass);
+                           pw, ws, pl, flags, orig, pass);
     return hipGetLastError();
 }
 
@@ -319,8 +321,9 @@ __global__ __launch_bounds__(256) void k_scatter_b(const double* __restrict__ xy
                                                    const RegionDesc* __restrict__ reg, double h, double hx,
                                                    const int32_t* __restrict__ cell_of,
                                                    const int32_t* __restrict__ start, int32_t* __restrict__ fill,
-                                                   double4* __restrict__ pw, float4* __restrict__ pl,
-                                                   uint32_t* __restrict__ flags, int32_t* __restrict__ orig, int pass)
+                                                   double4* __restrict__ pw, double* __restrict__ ws,
+                                                   float4* __restrict__ pl, uint32_t* __restrict__ flags,
+                                                   int32_t* __restrict__ orig, int pass)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int r = region_of(off, nreg, i);
@@ -329,7 +332,9 @@ __global__ __launch_bounds__(256) void k_scatter_b(const double* __restrict__ xy
         const int c = cell_of[i];
         const int pos = start[c] + atomicAdd(&fill[c], 1);
         const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
-        pw[pos] = make_double4(x, y, z, w ? w[i] : 1.0);
+        const double wi = w ? w[i] : 1.0;
+        pw[pos] = make_double4(x, y, z, wi);
+        ws[pos] = wi;
         const int lc = c - R.cell_base;
         const int cx = lc % R.dim[0], cy = (lc / R.dim[0]) % R.dim[1], cz = lc / (R.dim[0] * R.dim[1]);
         pl[pos] = make_float4((float)(x - (R.lo[0] + cx * hx)), (float)(y - (R.lo[1] + cy * h)),
@@ -389,11 +394,12 @@ hipError_t launch_cell_count_b(const double* xyz, int64_t n, const int32_t* off,
 hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* buf, int64_t n, const int32_t* off,
                             int nreg, const RegionDesc* reg, double h, double hx, const int32_t* cell_of,
                             const int32_t* start,
-                            int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st)
+                            int32_t* fill, double4* pw, double* ws, float4* pl, uint32_t* flags, int32_t* orig,
+                            hipStream_t st)
 {
     for (int pass = 0; pass < 2; ++pass)
         hipLaunchKernelGGL(k_scatter_b, dim3(blocks_for(n)), dim3(256), 0, st, xyz, w, buf, n, off, nreg, reg, h, hx,
-                           cell_of, start, fill, pw, pl, flags, orig, pass);
+                           cell_of, start, fill, pw, ws, pl, flags, orig, pass);
     return hipGetLastError();
 }
 

@@ -59,6 +59,7 @@ struct GsBatch {
     DevBuf<unsigned char> big;           // 3PCF: device-memory neighbour lists
     DevBuf<unsigned char> d_regions, params;
     DevBuf<double4> pw;
+    DevBuf<double> ws;                   // sorted raw weights
     DevBuf<float4> pl;
     DevBuf<uint32_t> flags;
     DevBuf<unsigned long long> out;
@@ -105,6 +106,7 @@ struct CtxView {
     double bbox[6];
     GridDesc g;
     const double4* pw;
+    const double* ws;           // sorted raw weights
     const uint32_t* flags;
     const int32_t* orig;
     const int32_t* cell_start;
@@ -128,6 +130,7 @@ int ctx_build_grid(gs_ctx* c, double rmax, int m, int sx, double* h_out);
 struct GridBufs {
     int32_t *cell_of, *cell_start, *fill, *chunk_nc, *chunk_off, *scan_tmp;
     double4* pw;
+    double* ws;
     float4* pl;
     uint32_t* flags;
     int32_t* orig;

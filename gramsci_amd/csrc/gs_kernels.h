@@ -20,7 +20,7 @@ hipError_t exclusive_scan(const int32_t* in, int32_t* out, int64_t n, int32_t* s
 int64_t scan_scratch_size(int64_t n);
 hipError_t launch_scatter(const double* xyz, const double* w, const int32_t* buf, int64_t n, int64_t n_data,
                           const GridDesc& g, const int32_t* cell_of, const int32_t* start, int32_t* fill,
-                          double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
+                          double4* pw, double* ws, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
 hipError_t launch_row_chunks(const int32_t* start, int nx, int nrow, int32_t* nch, hipStream_t st);
 
 // batched regions (grid.hip)
@@ -30,7 +30,8 @@ hipError_t launch_cell_count_b(const double* xyz, int64_t n, const int32_t* off,
 hipError_t launch_scatter_b(const double* xyz, const double* w, const int32_t* buf, int64_t n, const int32_t* off,
                             int nreg, const RegionDesc* reg, double h, double hx, const int32_t* cell_of,
                             const int32_t* start,
-                            int32_t* fill, double4* pw, float4* pl, uint32_t* flags, int32_t* orig, hipStream_t st);
+                            int32_t* fill, double4* pw, double* ws, float4* pl, uint32_t* flags, int32_t* orig,
+                            hipStream_t st);
 hipError_t launch_row_chunks_b(const int32_t* start, const RegionDesc* reg, const int32_t* row_base, int nreg,
                                int nrow_tot, int32_t* nch, hipStream_t st);
 
@@ -43,6 +44,7 @@ hipError_t launch_region_scatter(const float* xyz, int64_t n, const float* box, 
 // Arguments of the persistent 2PCF pair kernel (pair2.hip).
 struct PairArgs {
     const double4* pw;          // sorted points: x, y, z, raw weight
+    const double* ws;           // sorted raw weights alone (the weighted register tile reads 8 B per entry)
     const float4* pl;           // sorted: cell-local FP32 x, y, z and the cell x index
     const uint32_t* flags;      // sorted: bit0 random, bit1 never-apex
     const int32_t* cell_start;  // ncell+1
@@ -168,6 +170,7 @@ int quad4_occupancy(const QuadArgs& a, int nb, bool weighted);
 // Arguments of the persistent 2PCF adjoint kernel (adjoint.hip).
 struct AdjArgs {
     const double4* pw;
+    const double* ws;           // sorted raw weights alone
     const float4* pl;
     const uint32_t* flags;
     const int32_t* orig;

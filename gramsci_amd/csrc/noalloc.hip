@@ -8,7 +8,7 @@
 //
 // Limits: at most kNoAllocMaxPoints points and kNoAllocMaxCells grid cells
 // (the cell edge is widened until the grid fits).  Each GPU that runs the
-// library holds the array (about 50 MB) until the process ends.  Every call on
+// library holds the array (about 54 MB) until the process ends.  Every call on
 // a GPU uses that GPU's one array: calls that overlap on one GPU race.
 #include <cmath>
 #include <cstring>
@@ -31,7 +31,8 @@ constexpr size_t al(size_t b) { return (b + 255) / 256 * 256; }
 
 // byte offsets of the parts of the array
 constexpr size_t kOffPw = 0;
-constexpr size_t kOffPl = kOffPw + al(kNoAllocMaxPoints * sizeof(double4));
+constexpr size_t kOffWs = kOffPw + al(kNoAllocMaxPoints * sizeof(double4));
+constexpr size_t kOffPl = kOffWs + al(kNoAllocMaxPoints * sizeof(double));
 constexpr size_t kOffFlags = kOffPl + al(kNoAllocMaxPoints * sizeof(float4));
 constexpr size_t kOffOrig = kOffFlags + al(kNoAllocMaxPoints * 4);
 constexpr size_t kOffCellOf = kOffOrig + al(kNoAllocMaxPoints * 4);
@@ -162,8 +163,8 @@ extern "C" int gs_count_2pcf_noalloc(gs_ctx* c, const gs_params* p, const double
     if ((int)rows.size() > kMaxRows) return fail(GS_ELIMIT, "stencil too large");
     GridBufs gb{(int32_t*)(base + kOffCellOf), (int32_t*)(base + kOffCellStart), (int32_t*)(base + kOffFill),
                 (int32_t*)(base + kOffChunkNc), (int32_t*)(base + kOffChunkOff), (int32_t*)(base + kOffScan),
-                (double4*)(base + kOffPw), (float4*)(base + kOffPl), (uint32_t*)(base + kOffFlags),
-                (int32_t*)(base + kOffOrig)};
+                (double4*)(base + kOffPw), (double*)(base + kOffWs), (float4*)(base + kOffPl),
+                (uint32_t*)(base + kOffFlags), (int32_t*)(base + kOffOrig)};
     (void)hipEventRecord(v.ev[0], s);
     GridDesc g{};
     int32_t n_chunks = 0;
@@ -198,6 +199,7 @@ extern "C" int gs_count_2pcf_noalloc(gs_ctx* c, const gs_params* p, const double
 
     PairArgs a{};
     a.pw = gb.pw;
+    a.ws = gb.ws;
     a.pl = gb.pl;
     a.flags = gb.flags;
     a.cell_start = gb.cell_start;

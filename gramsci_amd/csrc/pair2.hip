@@ -392,7 +392,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         double ewj = 0.0;
                         if (!skip) {
                             e = a.pl[jl];
-                            if constexpr (W) ewj = ldexp(a.pw[jl].w, (ef & 1u) ? sr : sd);
+                            if constexpr (W) ewj = ldexp(a.ws[jl], (ef & 1u) ? sr : sd);
                         }
                         // entry index, class and buffer flag in one rotated register (-1: no entry):
                         // ep = eidx << 16 | (class offset of a random entry, nb*nm, or 0) << 1 | buffer,

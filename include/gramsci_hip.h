@@ -110,7 +110,7 @@ int gs_count_2pcf(gs_ctx* ctx, const gs_params* p, uint64_t* cnt, double* wsum,
 /* gs_count_2pcf for a catalogue in device memory (d_xyz [n][3], d_w [n] or
  * NULL, d_buf [n] or NULL; data first, then randoms), allocating nothing: the
  * sorted catalogue, grid, parameters, accumulators and counters live in one
- * file-scope device array of the library (about 50 MB per GPU, held until the
+ * file-scope device array of the library (about 54 MB per GPU, held until the
  * process ends).  No counterpart in the reference: for callers that cannot
  * allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc).  Limits: at most 524,288 points

@@ -142,7 +142,7 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool 
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
     const int waves = wg / kWave;
     if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // LDS tile: x, y, z, w, flags, pad
-    else if (!weighted) b += (size_t)waves * kWave * (16 + 4);  // LDS tile: float4, flags
+    else if (!weighted && nm > 1) b += (size_t)waves * kWave * (16 + 4);  // LDS tile: float4, flags
     return b;
 }
 
@@ -372,13 +372,15 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         finish(fj, W ? tw[kk] : 0.0, b, m, mr);
                     }
                 } else {
-                    if constexpr (W) {
+                    if constexpr (W || !RSD) {
                         // Register tile: lane L loads entry L; the entries then rotate through the
                         // wave one lane per step (DPP), so after 64 steps every lane has met every
                         // entry (after P steps for a tile of nt <= P entries, below).  Consecutive
                         // lanes meet different neighbours at each step, which spreads their histogram
                         // updates over many bins, and no LDS is read for the neighbours.  Used for
-                        // weighted sums, where it beats the LDS tile.
+                        // weighted sums and for unweighted counts without mu bins, where it beats the
+                        // LDS tile (the unweighted (r, mu) loop needs more registers than the LDS
+                        // tile's, which costs it a wave per SIMD).
                         // A partial tile is loaded with period P (the power of two >= nt): lane L
                         // holds entry L mod P, a full-wave rotation keeps that pattern, and P steps
                         // bring every entry to every lane.
@@ -411,6 +413,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                                 const float2 ok = okb[b];
                                 bool safe = (s >= ok.x) && (s < ok.y);
                                 int m = 0, mr;      // mr is read only after an FP64 recheck has set it
+                                if constexpr (!RSD) mr = 0;
                                 if constexpr (RSD) {
                                     const float r = __builtin_amdgcn_rsqf(s);
                                     // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             ep = rot_lane(ep);
                         }
                     } else {
-                        // Unweighted: LDS tile with the rotated schedule of the FP64 path (a partial
+                        // Unweighted (r, mu): LDS tile with the rotated schedule of the FP64 path (a partial
                         // tile costs only its nt steps; faster than the register tile when no weight
                         // product rides along).  eval32 bins the pair of this lane's point and tile
                         // entry eidx, with the guard band and the FP64 recheck.

@@ -10,8 +10,9 @@
 // of cells (fixed cy, cz) is one contiguous range.  A work item is a chunk of
 // 64 consecutive points of one x-row, one point per lane of a wavefront.  The
 // wave walks the half stencil of x-rows (dy, dz); for each it streams the
-// contiguous range of cells [cx_first + x0, cx_last + x1] through a
-// wave-private LDS tile of 64 points that every lane reads by broadcast.
+// contiguous range of cells [cx_first + x0, cx_last + x1] in tiles of 64
+// points: in FP32 a register tile that rotates through the wave by DPP, in
+// FP64 a wave-private LDS tile read in a rotated order.
 // Waves are persistent and independent (no work-group barriers in the main
 // loop); they pull items from 8 queues, one per XCD, each covering a
 // contiguous z-slab of the grid so that an XCD's L2 sees one slab, and steal
@@ -142,12 +143,13 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool 
     b += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;         // t64
     const int waves = wg / kWave;
     if (fp64) b += (size_t)waves * kWave * (24 + 8 + 4 + 4);  // LDS tile: x, y, z, w, flags, pad
-    else if (!weighted && nm > 1) b += (size_t)waves * kWave * (16 + 4);  // LDS tile: float4, flags
     return b;
 }
 
+// The unweighted FP32 kernels are compiled for six waves per SIMD (at most 80 VGPRs), the occupancy
+// their LDS use allows; the weighted ones are held to fewer waves by their LDS sums anyway.
 template <bool FP64, bool RSD, bool W, bool LOGB>
-__global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && !W) ? 6 : 1))) void k_pair2(PairArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const BinDesc& B = *a.bins;
@@ -177,11 +179,8 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
     p += (size_t)(kMaxBins + 1) * 8;
     double* t64 = (double*)p;
     p += ((size_t)(kMaxBins + 2) * 8 + 15) / 16 * 16;
-    // wave-private LDS tiles: FP64 x[64], y[64], z[64], w[64], flags[64];
-    // FP32 unweighted float4[64], flags[64]
-    unsigned char* tb = p + (size_t)wv * kWave * (FP64 ? 40 : 20);
-    float4* tl = (float4*)tb;
-    uint32_t* tlf = (uint32_t*)(tl + kWave);
+    // FP64: wave-private LDS tile x[64], y[64], z[64], w[64], flags[64]
+    unsigned char* tb = p + (size_t)wv * kWave * 40;
     double* tx = (double*)tb;
     double* ty = tx + kWave;
     double* tz = ty + kWave;
@@ -372,15 +371,12 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                         finish(fj, W ? tw[kk] : 0.0, b, m, mr);
                     }
                 } else {
-                    if constexpr (W || !RSD) {
+                    {
                         // Register tile: lane L loads entry L; the entries then rotate through the
                         // wave one lane per step (DPP), so after 64 steps every lane has met every
                         // entry (after P steps for a tile of nt <= P entries, below).  Consecutive
                         // lanes meet different neighbours at each step, which spreads their histogram
-                        // updates over many bins, and no LDS is read for the neighbours.  Used for
-                        // weighted sums and for unweighted counts without mu bins, where it beats the
-                        // LDS tile (the unweighted (r, mu) loop needs more registers than the LDS
-                        // tile's, which costs it a wave per SIMD).
+                        // updates over many bins, and no LDS is read for the neighbours.
                         // A partial tile is loaded with period P (the power of two >= nt): lane L
                         // holds entry L mod P, a full-wave rotation keeps that pattern, and P steps
                         // bring every entry to every lane.
@@ -465,75 +461,6 @@ __global__ __launch_bounds__(1024) void k_pair2(PairArgs a)
                             e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
                             if constexpr (W) ewj = rot_lane(ewj);
                             ep = rot_lane(ep);
-                        }
-                    } else {
-                        // Unweighted (r, mu): LDS tile with the rotated schedule of the FP64 path (a partial
-                        // tile costs only its nt steps; faster than the register tile when no weight
-                        // product rides along).  eval32 bins the pair of this lane's point and tile
-                        // entry eidx, with the guard band and the FP64 recheck.
-                        auto eval32 = [&](const float4 e, const uint32_t ef, const int eidx) {
-                            const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
-                            const float dy = e.y - yi_r, dz = e.z - zi_r;
-                            const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                            if (!(s > rej_lo && s < rej_hi)) return;
-                            int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
-                            const float2 ok = okb[b];
-                            bool safe = (s >= ok.x) && (s < ok.y);
-                            int m = 0, mr = 0;
-                            bool mirrored = true;
-                            if constexpr (RSD) {
-                                const float r = __builtin_amdgcn_rsqf(s);
-                                // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
-                                const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
-                                const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
-                                const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
-                                const float rm = __builtin_amdgcn_rsqf(M2);
-                                const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
-                                const float dt = fmaf(dt_ew, r, fmaf(dt_em, rm, dt_c));
-                                // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
-                                // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
-                                // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
-                                const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
-                                safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
-                                m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
-                            }
-                            if (!safe) {
-                                ++st.rechk;
-                                const double4 Pj = a.pw[jt + eidx];
-                                Pi = a.pw[i];
-                                const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
-                                if (!(se > t_lo && se < t_hi)) return;
-                                b = exact_bin(se, t64, nb);
-                                if (b == nb) { st.over += (uint64_t)(ci + ((ef & 1) ? 0 : 1)); return; }
-                                if constexpr (RSD) {
-                                    exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
-                                    mirrored = mr == nm - 1 - m;
-                                }
-                            }
-                            const int cj = (ef & 1) ? 0 : 1;
-                            const int base = cb0 + (int)(ef >> 1) + __umul24(b, nm);
-                            accumulate<RSD, false>(hcr, hw, nm, mirror, base, m,
-                                                   mirrored ? nm - 1 - m : mr, mirrored, ci, cj, 0.0,
-                                                   a.out_cnt, a.out_w);
-                        };
-                        // the tile's flags word holds the entry's class offset (nb*nm for a random, 0 for a
-                        // data point) shifted left by one, and its buffer bit in bit 0
-                        wave_sync();
-                        uint32_t fl = 2u;
-                        if (lane < nt) {
-                            tl[lane] = a.pl[jt + lane];
-                            fl = a.flags[jt + lane];
-                            tlf[lane] = ((fl & 1u) ? (uint32_t)nbm << 1 : 0u) | ((fl >> 1) & 1u);
-                        }
-                        const bool skip = chunk_buf && __ballot((fl & 2u) == 0) == 0;
-                        wave_sync();
-                        const int kend = (vi && !skip) ? nt : 0;
-                        int kk = lane % nt;        // this lane's entry at step k: (k + lane) mod nt
-                        for (int k = 0; k < kend; ++k) {
-                            const int kc = kk;
-                            kk = kk + 1 == nt ? 0 : kk + 1;
-                            if (kc < jmin) continue;
-                            eval32(tl[kc], tlf[kc], kc);
                         }
                     }
                 }

@@ -13,7 +13,7 @@
 //
 // One work-group per apex, persistent over apexes:
 //   1. gather the kept neighbours (exact FP64 bins) into LDS;
-//   2. sort them by original index (bitonic sort in LDS);
+//   2. sort them by original index (a rank sort in LDS; bitonic for device-memory lists);
 //   3. tabulate every neighbour-neighbour side bin in a K x K LDS matrix;
 //   4. enumerate linked (k1, k2) pairs over the lanes, each lane scanning k3.
 // Counts go to an LDS histogram [cell][16 random patterns] (or straight to
@@ -211,35 +211,50 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             continue;
         }
 
-        // ---- 2. bitonic sort of (key, slot) by original index
-        int np2 = 1;
-        while (np2 < K) np2 <<= 1;
-        for (int k = tid; k < np2; k += kWG) pk[k] = k;           // slot permutation
-        __syncthreads();
-        for (int size = 2; size <= np2; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int t = tid; t < np2 / 2; t += kWG) {
-                    const int lo = 2 * t - (t & (stride - 1));
-                    const int hi = lo + stride;
-                    const bool up = ((lo & size) == 0);
-                    const int sa = pk[lo], sb = pk[hi];
-                    const int ka = sa < K ? key[sa] : 0x7fffffff;
-                    const int kb = sb < K ? key[sb] : 0x7fffffff;
-                    if ((ka > kb) == up) { pk[lo] = sb; pk[hi] = sa; }
-                }
-                __syncthreads();
-            }
-        }
-        // gather into sorted order: slot pk[u] -> position u, staged through the link-matrix area
-        // (not in use yet; 5*K*8 <= kmax*kmax bytes) or a device-memory list's staging area
+        // ---- 2. sort by original index into the staging area (the link-matrix area, not in use yet:
+        // 5*K*8 <= kmax*kmax bytes, or a device-memory list's staging area)
         double* tmp = (double*)L;
-        for (int u = tid; u < K; u += kWG) {
-            const int sl = pk[u];
-            tmp[u] = nx_[sl];
-            tmp[K + u] = ny_[sl];
-            tmp[2 * K + u] = nz_[sl];
-            tmp[3 * K + u] = nw_[sl];
-            ((int*)(tmp + 4 * K))[u] = jl[sl];
+        if constexpr (!BIG) {
+            // K <= kmax <= kWG: one thread per entry; the original indices are distinct, so entry u goes
+            // to the number of keys below its own (all threads read key[j] together: LDS broadcasts)
+            for (int u = tid; u < K; u += kWG) {
+                const int ku = key[u];
+                int r = 0;
+                for (int j = 0; j < K; ++j) r += key[j] < ku ? 1 : 0;
+                tmp[r] = nx_[u];
+                tmp[K + r] = ny_[u];
+                tmp[2 * K + r] = nz_[u];
+                tmp[3 * K + r] = nw_[u];
+                ((int*)(tmp + 4 * K))[r] = jl[u];
+            }
+        } else {
+            // bitonic sort of (key, slot), then gather: slot pk[u] -> position u
+            int np2 = 1;
+            while (np2 < K) np2 <<= 1;
+            for (int k = tid; k < np2; k += kWG) pk[k] = k;           // slot permutation
+            __syncthreads();
+            for (int size = 2; size <= np2; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int t = tid; t < np2 / 2; t += kWG) {
+                        const int lo = 2 * t - (t & (stride - 1));
+                        const int hi = lo + stride;
+                        const bool up = ((lo & size) == 0);
+                        const int sa = pk[lo], sb = pk[hi];
+                        const int ka = sa < K ? key[sa] : 0x7fffffff;
+                        const int kb = sb < K ? key[sb] : 0x7fffffff;
+                        if ((ka > kb) == up) { pk[lo] = sb; pk[hi] = sa; }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int u = tid; u < K; u += kWG) {
+                const int sl = pk[u];
+                tmp[u] = nx_[sl];
+                tmp[K + u] = ny_[sl];
+                tmp[2 * K + u] = nz_[sl];
+                tmp[3 * K + u] = nw_[sl];
+                ((int*)(tmp + 4 * K))[u] = jl[sl];
+            }
         }
         __syncthreads();
         for (int u = tid; u < K; u += kWG) {

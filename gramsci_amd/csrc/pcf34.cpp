@@ -482,6 +482,7 @@ static int count4(gs_ctx* c, const gs_params* p, bool tetra, uint64_t* cnt, doub
     // device-memory lists, so no neighbour count is refused.
     const size_t budget = 160 * 1024;
     int kmax = 256;
+    static_assert(kWG >= 256, "the LDS-list sort of k_quad4 takes one thread per list entry");
     bool lds_hist = (size_t)ncell * 64 <= 16 * 1024;
     while (kmax > 64 && quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra, (int)rows.size()) > budget) kmax /= 2;
     if (quad4_lds_bytes(nb, ncell, kmax, lds_hist, tetra, (int)rows.size()) > budget) {

@@ -211,24 +211,30 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
             continue;
         }
 
-        // ---- 2. sort by original index into the staging area (the link-matrix area, not in use yet:
-        // 5*K*8 <= kmax*kmax bytes, or a device-memory list's staging area)
-        double* tmp = (double*)L;
+        // ---- 2. sort by original index
         if constexpr (!BIG) {
-            // K <= kmax <= kWG: one thread per entry; the original indices are distinct, so entry u goes
-            // to the number of keys below its own (all threads read key[j] together: LDS broadcasts)
-            for (int u = tid; u < K; u += kWG) {
-                const int ku = key[u];
-                int r = 0;
+            // K <= kmax <= kWG: one thread per entry.  The original indices are distinct, so entry u goes
+            // to the number of keys below its own (all threads read key[j] together: LDS broadcasts); the
+            // entry is held in registers across the barrier and written to its sorted place
+            const bool own = tid < K;
+            int r = 0, vj = 0;
+            double vx = 0.0, vy = 0.0, vz = 0.0, vw = 0.0;
+            if (own) {
+                const int ku = key[tid];
                 for (int j = 0; j < K; ++j) r += key[j] < ku ? 1 : 0;
-                tmp[r] = nx_[u];
-                tmp[K + r] = ny_[u];
-                tmp[2 * K + r] = nz_[u];
-                tmp[3 * K + r] = nw_[u];
-                ((int*)(tmp + 4 * K))[r] = jl[u];
+                vx = nx_[tid]; vy = ny_[tid]; vz = nz_[tid]; vw = nw_[tid];
+                vj = jl[tid];
             }
+            __syncthreads();
+            if (own) {
+                nx_[r] = vx; ny_[r] = vy; nz_[r] = vz; nw_[r] = vw;
+                pk[r] = vj;
+            }
+            __syncthreads();
         } else {
-            // bitonic sort of (key, slot), then gather: slot pk[u] -> position u
+            // bitonic sort of (key, slot), then gather (slot pk[u] -> position u) through the device-memory
+            // list's staging area
+            double* tmp = (double*)L;
             int np2 = 1;
             while (np2 < K) np2 <<= 1;
             for (int k = tid; k < np2; k += kWG) pk[k] = k;           // slot permutation
@@ -255,13 +261,13 @@ __global__ __launch_bounds__(kWG) void k_quad4(QuadArgs a)
                 tmp[3 * K + u] = nw_[sl];
                 ((int*)(tmp + 4 * K))[u] = jl[sl];
             }
+            __syncthreads();
+            for (int u = tid; u < K; u += kWG) {
+                nx_[u] = tmp[u]; ny_[u] = tmp[K + u]; nz_[u] = tmp[2 * K + u]; nw_[u] = tmp[3 * K + u];
+                pk[u] = ((int*)(tmp + 4 * K))[u];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int u = tid; u < K; u += kWG) {
-            nx_[u] = tmp[u]; ny_[u] = tmp[K + u]; nz_[u] = tmp[2 * K + u]; nw_[u] = tmp[3 * K + u];
-            pk[u] = ((int*)(tmp + 4 * K))[u];
-        }
-        __syncthreads();
 
         // ---- 3. link matrix of neighbour-neighbour side bins (find_dist); a device-memory list
         // evaluates them where they are needed instead

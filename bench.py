@@ -248,7 +248,10 @@ def bench_rmu(args, legs=True):
                      precision="FP32 + FP64 guard-band recheck" if main_prec == "fp32" else "plain FP64")
     line["roofline"] = {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                         "frac": achieved / peak, "traffic": None,
-                        "kernel": f"k_pair2<{'true' if main_prec == 'fp64' else 'false'}, true, true, true>",
+                        # the FP32 kernel on this many bins runs two apexes per lane (the fifth argument),
+                        # which the library picks where it costs no resident waves
+                        "kernel": ("k_pair2<true, true, true, true, false>" if main_prec == "fp64"
+                                   else "k_pair2<false, true, true, true, true>"),
                         "kernel_ms": R["t_kern"], "flops_per_pair": FLOPS_PER_PAIR_RMU}
     if main_prec == "fp32":
         c = pair_counters(R["t_kern"])

@@ -41,6 +41,8 @@
 // bit 32 (no carries, so no returning atomics).  A wave that has issued 2^29
 // increments drains the histogram with atomic exchanges into the global
 // 128-bit accumulators; the rest is drained when the work-group ends.
+#include <cstdlib>
+
 #include "gs_device.h"
 #include "gs_kernels.h"
 
@@ -146,9 +148,19 @@ size_t pair2_lds_bytes(int nb, int nm, int reps, bool fp64, bool weighted, bool 
     return b;
 }
 
+// a lane's run of histogram increments to one bin, merged in registers
+struct RunCache {
+    int ck;
+    uint32_t cc;
+    unsigned long long cq;
+};
+
 // The unweighted FP32 kernels are compiled for six waves per SIMD (at most 80 VGPRs), the occupancy
 // their LDS use allows; the weighted ones are held to fewer waves by their LDS sums anyway.
-template <bool FP64, bool RSD, bool W, bool LOGB>
+// DUAL (weighted FP32 only): each lane also holds the chunk point 32 lanes away, and a full tile is
+// done in 32 steps of two pairs each (below).  It needs about 127 VGPRs, so it is launched only where
+// that costs no resident waves.
+template <bool FP64, bool RSD, bool W, bool LOGB, bool DUAL>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && !W) ? 6 : 1))) void k_pair2(PairArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -228,24 +240,28 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && 
     // selects, not branches, so that no register copies are needed where paths merge.  An empty
     // run (cc = 0) stays attached to its last bin (bin 0 at the start): flushing it adds zeros, so
     // the flush needs no test for a missing run.
-    int ck = 0;
-    uint32_t cc = 0;
-    unsigned long long cq = 0;
-    auto flush = [&]() {
-        atomicAdd(&hcr[ck], cc);
+    static_assert(!DUAL || (W && !FP64), "two apexes per lane: weighted FP32 kernels only");
+    // with DUAL each lane's second apex has a run cache of its own
+    RunCache rc0{0, 0u, 0ull}, rc1{0, 0u, 0ull};
+    auto flush_one = [&](RunCache& c) {
+        atomicAdd(&hcr[c.ck], c.cc);
         if constexpr (W) {
-            atomicAdd(&hw[2 * ck], cq & 0xffffffffull);
-            atomicAdd(&hw[2 * ck + 1], cq >> 32);
+            atomicAdd(&hw[2 * c.ck], c.cq & 0xffffffffull);
+            atomicAdd(&hw[2 * c.ck + 1], c.cq >> 32);
         }
-        cc = 0;
-        cq = 0;
+        c.cc = 0;
+        c.cq = 0;
     };
-    auto put = [&](int k, uint32_t c, unsigned long long q) {   // c in {1, 2}
-        const bool nw = (k != ck) || (cc >= 32u);
-        if (nw) flush();
-        cc = (nw ? 0u : cc) + c;
-        if constexpr (W) cq = (nw ? 0ull : cq) + (q << (c - 1));
-        ck = k;
+    auto flush = [&]() {
+        flush_one(rc0);
+        if constexpr (DUAL) flush_one(rc1);
+    };
+    auto put = [&](RunCache& rc, int k, uint32_t c, unsigned long long q) {   // c in {1, 2}
+        const bool nw = (k != rc.ck) || (rc.cc >= 32u);
+        if (nw) flush_one(rc);
+        rc.cc = (nw ? 0u : rc.cc) + c;
+        if constexpr (W) rc.cq = (nw ? 0ull : rc.cq) + (q << (c - 1));
+        rc.ck = k;
     };
     unsigned long long bound = 0;   // LDS increments issued by this wave since its last drain
     // LDS bin increments between drains, over all waves sharing the histogram, stay below 2^31,
@@ -302,12 +318,27 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && 
             fi = a.flags[i];
         }
         const int ri = fi & 1, ci = (fi & 2) ? 0 : 1;
+        // the lane's second apex: the chunk point 32 lanes away
+        const int i2 = i0 + (lane ^ 32);
+        const bool vi2 = DUAL && (lane ^ 32) < n_i;
+        double4 Pi2 = make_double4(0, 0, 0, 0);
+        float4 Li2 = make_float4(0, 0, 0, 0);
+        uint32_t fi2 = 2u;
+        if (vi2) {
+            Pi2 = a.pw[i2];
+            Li2 = a.pl[i2];
+            fi2 = a.flags[i2];
+        }
+        const int ri2 = fi2 & 1, ci2 = (fi2 & 2) ? 0 : 1;
         // a chunk of buffer points only (never apexes) gains nothing from a tile of buffer points only
         const bool chunk_buf = __ballot(ci != 0) == 0;
         // the apex weight scaled by its class's 2^F (exact): a pair's product w_i' w_j' then carries
         // the scale of its class (DD, DR or RR)
         const double wis = ldexp(Pi.w, ri ? sr : sd);
         const float ax32 = (float)Pi.x, ay32 = (float)Pi.y, az32 = (float)Pi.z;   // absolute, for mu
+        const double wis2 = DUAL ? ldexp(Pi2.w, ri2 ? sr : sd) : 0.0;
+        const float ax2 = (float)Pi2.x, ay2 = (float)Pi2.y, az2 = (float)Pi2.z;
+        const int cb02 = ri2 ? nbm : 0;
         // histogram class offsets of this lane's pairs with a data / a random neighbour
         const int cb0 = ri ? nbm : 0, cb1 = cb0 + nbm;
 
@@ -325,9 +356,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && 
             // lane position relative to the neighbour row's cell origin in y and z
             const float yi_r = Li.y - (float)rw.dy * h32;
             const float zi_r = Li.z - (float)rw.dz * h32;
+            const float yi_r2 = Li2.y - (float)rw.dy * h32;
+            const float zi_r2 = Li2.z - (float)rw.dz * h32;
             for (int jt = js; jt < je; jt += kWave) {
                 const int nt = min(kWave, je - jt);
                 const int jmin = max(self ? i + 1 - jt : 0, 0);   // self row: only j > i
+                const int jmin2 = max(self ? i2 + 1 - jt : 0, 0);
                 if (vi) st.cand += (unsigned long long)(nt - min(jmin, nt));
                 bound += (unsigned long long)nt * 2ull * kWave;
                 // histogram update of one kept pair with neighbour flags fj, bin b, mu bins m / mr
@@ -397,70 +431,159 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && 
                         // so the self-row test eidx >= jmin is ep >= jmin << 16 and the pair's class
                         // offset is cb0 + a field of ep (nb*nm <= 127*127 fits in 15 bits)
                         int ep = has ? (le << 16) | ((ef & 1u) ? nbm << 1 : 0) | (int)((ef >> 1) & 1u) : -1;
-                        const int jmin16 = jmin << 16;
-                        const int kend = __builtin_amdgcn_readfirstlane(skip ? 0 : P);
-                        for (int k = 0; k < kend; ++k) {
-                            if (vi && ep >= jmin16) do {
-                                const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
-                                const float dy = e.y - yi_r, dz = e.z - zi_r;
-                                const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                                if (!(s > rej_lo && s < rej_hi)) break;
-                                int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
-                                const float2 ok = okb[b];
-                                bool safe = (s >= ok.x) && (s < ok.y);
-                                int m = 0, mr;      // mr is read only after an FP64 recheck has set it
-                                if constexpr (!RSD) mr = 0;
-                                if constexpr (RSD) {
-                                    const float r = __builtin_amdgcn_rsqf(s);
-                                    // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
-                                    const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
-                                    const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
-                                    const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
-                                    const float rm = __builtin_amdgcn_rsqf(M2);
-                                    const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
-                                    const float dt = fmaf(dt_ew, r, fmaf(dt_em, rm, dt_c));
-                                    // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
-                                    // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
-                                    // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
-                                    const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
-                                    safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
-                                    m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
-                                }
-                                bool mirrored = true;     // the FP32 mu bins are mirror images by construction
-                                if (!safe) {
-                                    ++st.rechk;
-                                    const double4 Pj = a.pw[jt + (ep >> 16)];
-                                    Pi = a.pw[i];
-                                    const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
-                                    if (!(se > t_lo && se < t_hi)) break;
-                                    b = exact_bin(se, t64, nb);
-                                    if (b == nb) { st.over += (uint64_t)(ci + (ep & 1 ? 0 : 1)); break; }
+                        if constexpr (!DUAL) {
+                            const int jmin16 = jmin << 16;
+                            const int kend = __builtin_amdgcn_readfirstlane(skip ? 0 : P);
+                            for (int k = 0; k < kend; ++k) {
+                                if (vi && ep >= jmin16) do {
+                                    const float dx = fmaf(e.w - Li.w, hx32, e.x - Li.x);
+                                    const float dy = e.y - yi_r, dz = e.z - zi_r;
+                                    const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                                    if (!(s > rej_lo && s < rej_hi)) break;
+                                    int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
+                                    const float2 ok = okb[b];
+                                    bool safe = (s >= ok.x) && (s < ok.y);
+                                    int m = 0, mr;      // mr is read only after an FP64 recheck has set it
+                                    if constexpr (!RSD) mr = 0;
                                     if constexpr (RSD) {
-                                        exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
-                                        mirrored = mr == nm - 1 - m;
+                                        const float r = __builtin_amdgcn_rsqf(s);
+                                        // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
+                                        const float mx = fmaf(0.5f, dx, ax32), my = fmaf(0.5f, dy, ay32), mz = fmaf(0.5f, dz, az32);
+                                        const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
+                                        const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
+                                        const float rm = __builtin_amdgcn_rsqf(M2);
+                                        const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
+                                        const float dt = fmaf(dt_ew, r, fmaf(dt_em, rm, dt_c));
+                                        // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
+                                        // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
+                                        // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
+                                        const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
+                                        safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
+                                        m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
                                     }
-                                }
-                                {
-                                    const int cj = (ep & 1) ? 0 : 1;
-                                    const int base = cb0 + ((ep >> 1) & 0x7fff) + __umul24(b, nm);
-                                    unsigned long long q = 0;
-                                    if constexpr (W) q = quantise_rn(wis * ewj);
-                                    if constexpr (W && !RSD) {
-                                        if (ci + cj) put(base, (uint32_t)(ci + cj), q);
-                                    } else if (W && mirror && mirrored) {
-                                        put(base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
-                                    } else {
-                                        // mr: the mirror image of m unless the recheck found otherwise
-                                        accumulate<RSD, W>(hcr, hw, nm, mirror, base, m,
-                                                           mirrored ? nm - 1 - m : mr, mirrored, ci, cj,
-                                                           wis * ewj, a.out_cnt, a.out_w);
+                                    bool mirrored = true;     // the FP32 mu bins are mirror images by construction
+                                    if (!safe) {
+                                        ++st.rechk;
+                                        const double4 Pj = a.pw[jt + (ep >> 16)];
+                                        Pi = a.pw[i];
+                                        const double se = exact_s(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z);
+                                        if (!(se > t_lo && se < t_hi)) break;
+                                        b = exact_bin(se, t64, nb);
+                                        if (b == nb) { st.over += (uint64_t)(ci + (ep & 1 ? 0 : 1)); break; }
+                                        if constexpr (RSD) {
+                                            exact_mu(Pi.x, Pi.y, Pi.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                            mirrored = mr == nm - 1 - m;
+                                        }
                                     }
+                                    {
+                                        const int cj = (ep & 1) ? 0 : 1;
+                                        const int base = cb0 + ((ep >> 1) & 0x7fff) + __umul24(b, nm);
+                                        unsigned long long q = 0;
+                                        if constexpr (W) q = quantise_rn(wis * ewj);
+                                        if constexpr (W && !RSD) {
+                                            if (ci + cj) put(rc0, base, (uint32_t)(ci + cj), q);
+                                        } else if (W && mirror && mirrored) {
+                                            put(rc0, base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
+                                        } else {
+                                            // mr: the mirror image of m unless the recheck found otherwise
+                                            accumulate<RSD, W>(hcr, hw, nm, mirror, base, m,
+                                                               mirrored ? nm - 1 - m : mr, mirrored, ci, cj,
+                                                               wis * ewj, a.out_cnt, a.out_w);
+                                        }
+                                    }
+                                } while (false);
+                                // rotate the register tile by one lane (all lanes active here)
+                                e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
+                                if constexpr (W) ewj = rot_lane(ewj);
+                                ep = rot_lane(ep);
+                            }
+                        } else {
+                            const int kend = __builtin_amdgcn_readfirstlane(skip ? 0 : P);
+                            // one apex's pair with the lane's current tile entry
+                            auto visit = [&](const bool va, const int ia, const float4 La, const float yr, const float zr,
+                                             const float axa, const float aya, const float aza, const int cia,
+                                             const int cba, const double wisa, const int jm16,
+                                             RunCache& rc) __attribute__((always_inline)) {
+                                if (va && ep >= jm16) do {
+                                    const float dx = fmaf(e.w - La.w, hx32, e.x - La.x);
+                                    const float dy = e.y - yr, dz = e.z - zr;
+                                    const float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                                    if (!(s > rej_lo && s < rej_hi)) break;
+                                    int b = (int)__builtin_amdgcn_fmed3f(bin_estimate(s, LOGB, ua, ubv), 0.0f, nbm1f);
+                                    const float2 ok = okb[b];
+                                    bool safe = (s >= ok.x) && (s < ok.y);
+                                    int m = 0, mr;      // mr is read only after an FP64 recheck has set it
+                                    if constexpr (!RSD) mr = 0;
+                                    if constexpr (RSD) {
+                                        const float r = __builtin_amdgcn_rsqf(s);
+                                        // mid = x_i + d/2 (absolute), sep = x_i - x_j = -d
+                                        const float mx = fmaf(0.5f, dx, axa), my = fmaf(0.5f, dy, aya), mz = fmaf(0.5f, dz, aza);
+                                        const float D = -fmaf(mz, dz, fmaf(my, dy, mx * dx));
+                                        const float M2 = fmaf(mz, mz, fmaf(my, my, mx * mx));
+                                        const float rm = __builtin_amdgcn_rsqf(M2);
+                                        const float t = fmaf(D * rm * r, ms32, ms32);     // (theta + 1) * mu_scale
+                                        const float dt = fmaf(dt_ew, r, fmaf(dt_em, rm, dt_c));
+                                        // mu edges sit at the integers 1..nm-1 of t: the bin is certain when the
+                                        // nearest edge, clamp(rint(t), 1, nm-1), is farther than the error bound
+                                        // dt < 1/2 (a nearest integer of 0 or nm is more than 1/2 from any edge)
+                                        const float edge = __builtin_amdgcn_fmed3f(rintf(t), e_lo, e_hi);
+                                        safe = safe && (dt < 0.5f) && (fabsf(t - edge) >= dt);
+                                        m = min((int)t, nm - 1);       // t >= 0 up to rounding; truncation = floor there
+                                    }
+                                    bool mirrored = true;     // the FP32 mu bins are mirror images by construction
+                                    if (!safe) {
+                                        ++st.rechk;
+                                        const double4 Pj = a.pw[jt + (ep >> 16)];
+                                        const double4 Pa = a.pw[ia];
+                                        const double se = exact_s(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z);
+                                        if (!(se > t_lo && se < t_hi)) break;
+                                        b = exact_bin(se, t64, nb);
+                                        if (b == nb) { st.over += (uint64_t)(cia + (ep & 1 ? 0 : 1)); break; }
+                                        if constexpr (RSD) {
+                                            exact_mu(Pa.x, Pa.y, Pa.z, Pj.x, Pj.y, Pj.z, ms, nm, m, mr);
+                                            mirrored = mr == nm - 1 - m;
+                                        }
+                                    }
+                                    {
+                                        const int cj = (ep & 1) ? 0 : 1;
+                                        const int base = cba + ((ep >> 1) & 0x7fff) + __umul24(b, nm);
+                                        unsigned long long q = 0;
+                                        if constexpr (W) q = quantise_rn(wisa * ewj);
+                                        if constexpr (W && !RSD) {
+                                            if (cia + cj) put(rc, base, (uint32_t)(cia + cj), q);
+                                        } else if (W && mirror && mirrored) {
+                                            put(rc, base + m, 1u, q);   // stands for i->j at m and j->i at nm-1-m
+                                        } else {
+                                            // mr: the mirror image of m unless the recheck found otherwise
+                                            accumulate<RSD, W>(hcr, hw, nm, mirror, base, m,
+                                                               mirrored ? nm - 1 - m : mr, mirrored, cia, cj,
+                                                               wisa * ewj, a.out_cnt, a.out_w);
+                                        }
+                                    }
+                                } while (false);
+                            };
+                            auto rotate = [&]() __attribute__((always_inline)) {
+                                // rotate the register tile by one lane (all lanes active here)
+                                e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
+                                if constexpr (W) ewj = rot_lane(ewj);
+                                ep = rot_lane(ep);
+                            };
+                            if (kend == kWave) {
+                                // a full tile in 32 steps: lane L meets 32 cyclically consecutive entries from
+                                // its own, lane L ^ 32 the other 32, and each pairs them with both apexes,
+                                // points L and L ^ 32, so every (apex, entry) pair is visited exactly once
+                                // and each rotation serves two pairs
+                                for (int k = 0; k < kWave / 2; ++k) {
+                                    visit(vi, i, Li, yi_r, zi_r, ax32, ay32, az32, ci, cb0, wis, jmin << 16, rc0);
+                                    visit(vi2, i2, Li2, yi_r2, zi_r2, ax2, ay2, az2, ci2, cb02, wis2, jmin2 << 16, rc1);
+                                    rotate();
                                 }
-                            } while (false);
-                            // rotate the register tile by one lane (all lanes active here)
-                            e.x = rot_lane(e.x); e.y = rot_lane(e.y); e.z = rot_lane(e.z); e.w = rot_lane(e.w);
-                            if constexpr (W) ewj = rot_lane(ewj);
-                            ep = rot_lane(ep);
+                            } else {
+                                for (int k = 0; k < kend; ++k) {
+                                    visit(vi, i, Li, yi_r, zi_r, ax32, ay32, az32, ci, cb0, wis, jmin << 16, rc0);
+                                    rotate();
+                                }
+                            }
                         }
                     }
                 }
@@ -499,10 +622,36 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu((!FP64 && 
     }
 }
 
+template <bool FP64, bool RSD, bool W, bool LOGB, bool DUAL>
+static int occ_k(size_t lds, int wg)
+{
+    int nb = 0;
+    auto kern = k_pair2<FP64, RSD, W, LOGB, DUAL>;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, wg, lds) != hipSuccess)
+        return 1;
+    return nb;
+}
+
+// Whether the weighted FP32 kernel runs with two apexes per lane: where the LDS histogram already limits
+// the resident work-groups to what the larger register file allows (many bins), else not.
+// GS_PAIR_DUAL=0 / 1 forces it off / on (tests, tuning).
+template <bool RSD, bool LOGB>
+static bool use_dual(size_t lds, int wg)
+{
+    const char* ev = getenv("GS_PAIR_DUAL");
+    if (ev && *ev) return atoi(ev) != 0;
+    return occ_k<false, RSD, true, LOGB, true>(lds, wg) >= occ_k<false, RSD, true, LOGB, false>(lds, wg);
+}
+
 template <bool FP64, bool RSD, bool W, bool LOGB>
 static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, int wg, hipStream_t st)
 {
-    auto kern = k_pair2<FP64, RSD, W, LOGB>;
+    auto kern = k_pair2<FP64, RSD, W, LOGB, false>;
+    if constexpr (W && !FP64)
+        if (use_dual<RSD, LOGB>(lds, wg)) kern = k_pair2<FP64, RSD, W, LOGB, true>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -514,14 +663,7 @@ static hipError_t launch_t(const PairArgs& a, size_t lds, int nblocks, int wg, h
 template <bool FP64, bool RSD, bool W, bool LOGB>
 static int occ_t(size_t lds, int wg)
 {
-    int nb = 0;
-    auto kern = k_pair2<FP64, RSD, W, LOGB>;
-    if (lds > 64 * 1024 &&
-        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, wg, lds) != hipSuccess)
-        return 1;
-    return nb;
+    return occ_k<FP64, RSD, W, LOGB, false>(lds, wg);
 }
 
 // Runs F<FP64, RSD, W, LOGB>(args...) for the runtime flags.

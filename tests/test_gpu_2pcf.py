@@ -166,6 +166,27 @@ def test_x_cell_subdivisions(eng, monkeypatch, sx):
     _check(eng, C.lattice(11, 1.0, -5.0), None, 11 ** 3, O.Params(1.0, 4.0, 3, 4, False))
 
 
+@pytest.mark.parametrize("dual", ["0", "1"])
+def test_two_apexes_per_lane(eng, monkeypatch, dual):
+    # the weighted FP32 kernel with and without its two-apexes-per-lane schedule (GS_PAIR_DUAL forces
+    # it; unset, it runs where it costs no resident waves): counts, N2/N3 and weighted sums match the
+    # oracle, with buffer points, ragged rows, (r, mu) log bins and plain linear bins from r = 0, and the
+    # FP32 sums equal the FP64 path's bit for bit
+    monkeypatch.setenv("GS_PAIR_DUAL", dual)
+    rng = np.random.default_rng(61)
+    d = C.clustered(7000, 130.0, 62)
+    r = C.uniform(14000, 130.0, 63)
+    xyz = np.concatenate([d, r])
+    w = np.concatenate([C.lognormal_weights(7000, 0.3, 64), np.ones(14000)])
+    buf = (rng.random(len(xyz)) < 0.15).astype(np.int32)
+    print(f"dual={dual}", flush=True)
+    for p, b, cds in ((O.Params(1.0, 25.0, 9, 6, True), buf, (0, 2)),
+                      (O.Params(1.0, 25.0, 9, 6, True), None, (0, 3)),
+                      (O.Params(0.0, 20.0, 7, 1, False), buf, (0, 3))):
+        _, outs = _check(eng, xyz, w, 7000, p, buf=b, cell_divs=cds)
+        assert np.array_equal(outs[0]["wsum"], outs[len(cds)]["wsum"])
+
+
 def test_lattice_edges_and_origin(eng):
     # exact-edge distances and mu values on a lattice that contains the origin
     # (mid = 0 pairs give theta = NaN in the reference)
